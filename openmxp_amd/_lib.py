@@ -1,0 +1,74 @@
+"""ctypes binding of libomxp.so (the C ABI declared in include/omxp.h).
+
+This module only loads the native library and declares argument types; it
+has no fallback.  If libomxp.so is missing the import of a function raises,
+so a GPU run can never silently fall back to host code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libomxp.so")
+
+i64 = C.c_int64
+u64 = C.c_uint64
+i32 = C.c_int
+dbl = C.c_double
+flt = C.c_float
+vp = C.c_void_p
+sz = C.c_size_t
+
+# name -> (restype, argtypes); kept in the order of include/omxp.h
+SIGNATURES = {
+    "omx_version": (C.c_char_p, []),
+    "omx_matgen_f32": (i32, [i64, u64, vp, i64, vp, vp]),
+    "omx_matgen_block_f32": (i32, [i64, u64, i64, i64, i64, i64, vp, i64, vp]),
+    "omx_diag_f64": (i32, [i64, u64, i64, i64, vp, vp]),
+    "omx_rhs_f64": (i32, [i64, u64, vp, vp]),
+    "omx_otf_gemv_workspace_bytes": (sz, [i64]),
+    "omx_otf_gemv_f64": (i32, [i64, u64, vp, dbl, vp, vp, vp, sz, vp]),
+    "omx_norm_inf_f64": (i32, [i64, vp, vp, vp]),
+    "omx_copy_f64": (i32, [i64, vp, vp, vp]),
+    "omx_div_f64": (i32, [i64, vp, vp, vp, vp]),
+    "omx_axpy_f64": (i32, [i64, dbl, vp, vp, vp]),
+    "omx_trsv_workspace_bytes": (sz, [i64]),
+    "omx_trsv_lower_unit_f32f64": (i32, [i64, i64, vp, i64, vp, vp, sz, vp]),
+    "omx_trsv_upper_f32f64": (i32, [i64, i64, vp, i64, vp, vp, sz, vp]),
+    "omx_harness": (dbl, [dbl, dbl, dbl, dbl, i64]),
+}
+
+_lib = None
+
+
+class OmxError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libomxp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OmxError(
+                f"{LIB_PATH} is missing: build it with `python -m openmxp_amd.build` "
+                "(the HIP path has no fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            if not hasattr(L, name):
+                continue
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    return list(SIGNATURES)
+
+
+def check(rc: int, what: str = "omx call") -> None:
+    if rc != 0:
+        raise OmxError(f"{what} returned {rc}")

@@ -1,0 +1,49 @@
+"""The C ABI library loads and exports every symbol include/omxp.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "omxp.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(omx_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert "omx_matgen_f32" in names and len(names) >= 10
+
+
+def test_library_exports_every_declared_symbol():
+    from openmxp_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libomxp.so is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in _declared() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    from openmxp_amd import _lib
+    assert sorted(_lib.declared_symbols()) == _declared()
+
+
+def test_version_and_host_harness():
+    from openmxp_amd import _lib
+    L = _lib.lib()
+    assert L.omx_version().startswith(b"omxp")
+    assert L.omx_harness(1e-10, 2.0, 3.0, 0.5, 1000) == 1e-10 / 6.5 / (1000 * 2.0**-53)
+
+
+def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+    from openmxp_amd import _lib
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(_lib.OmxError):
+        _lib.lib()
