@@ -34,6 +34,18 @@ extern "C" {
 #define OMX_ERR_WORKSPACE (-3) /* workspace too small */
 #define OMX_ERR_NOCONV (-4)    /* refinement did not converge */
 
+/* Masks of omx_pack_f16, on packed coordinates (r = panel row, k = column
+ * within the block): the -alt 2 triangular copies of the inverse factors. */
+#define OMX_MASK_NONE 0
+#define OMX_MASK_K_LE_R 1      /* keep k <= r, zero elsewhere            */
+#define OMX_MASK_K_GT_R_UNIT 2 /* keep k > r, 1 on k == r, zero elsewhere */
+#define OMX_MASK_R_GT_K_UNIT 3 /* keep r > k, 1 on r == k, zero elsewhere */
+
+/* Triangular-solve variants of omx_trsm_f32 (T is the triangle). */
+#define OMX_TRSM_RUNN 0 /* B <- B * T^-1, T upper, non-unit  */
+#define OMX_TRSM_LLNU 1 /* B <- T^-1 * B, T lower, unit      */
+#define OMX_TRSM_RLNU 2 /* B <- B * T^-1, T lower, unit      */
+
 /* Library version string (static storage). */
 const char *omx_version(void);
 
@@ -119,6 +131,149 @@ int omx_trsv_lower_unit_f32f64(int64_t n, int64_t b, const float *A,
 int omx_trsv_upper_f32f64(int64_t n, int64_t b, const float *A, int64_t lda,
                           double *x, void *work, size_t work_bytes,
                           void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Panel packing and the MFMA Schur GEMM.
+ *
+ * A panel is an R x K operand (K = the block size b).  Packed panels use the
+ * tile layout the GEMM streams: rows in strips of 256; each (strip, K-stage)
+ * is one contiguous 32 KB chunk (64 columns for FP16, 128 for FP8); inside
+ * it the data are ordered as the lane operands of v_mfma_f32_32x32x16_f16 /
+ * v_mfma_scale_f32_32x32x64_f8f6f4 (openmxp_amd/csrc/omx_layout.hpp gives
+ * the exact offset of element (r, k)).  Rows beyond R are zero.
+ * ------------------------------------------------------------------------- */
+
+/* Bytes of a packed R x K panel (fp8 = 0: FP16, 1: FP8 e4m3). */
+size_t omx_panel_bytes(int64_t rows, int64_t k, int fp8);
+
+/* Number of power-of-two exponents (one per 256 x 256 tile) of an FP8
+ * panel. */
+size_t omx_panel_exps_count(int64_t rows, int64_t k);
+
+/* Pack P = fp16(S) with S(r, l) = src[r*rs + l*ls], r < rows, l < k
+ * (k % 64 == 0), round to nearest even, optional triangular mask
+ * (OMX_MASK_*).  rs = 1, ls = lda packs a column panel A[k+1:, k]; rs = lda,
+ * ls = 1 packs a row panel A[k, k+1:] transposed.  chunk_max (optional,
+ * ceil(rows/256) * k/64 floats) receives max |S| per (strip, 64 columns).
+ * Replaces half_conversion_left, half_conversion_right_trans and the
+ * downCast_copy_* / downCast_trans_general launchers
+ * (fp16_gpu_kernels.h:26-49, fp16_gpu_kernels.cpp:11-303). */
+int omx_pack_f16(int64_t rows, int64_t k, const float *src, int64_t rs,
+                 int64_t ls, int mask, uint16_t *P, float *chunk_max,
+                 void *stream);
+
+/* Pack P = e4m3(S * 2^e) with one exponent e per 256 x 256 tile, the
+ * largest with max|tile| * 2^e <= 448 (k % 256 == 0); exps receives e and
+ * tile_max the tile max-abs (omx_panel_exps_count entries each, strip
+ * major).  No reference counterpart: the reference's conversion takes an
+ * unused `scale` argument (panel_trf.hpp:259, :306). */
+int omx_pack_f8(int64_t rows, int64_t k, const float *src, int64_t rs,
+                int64_t ls, uint8_t *P, int32_t *exps, float *tile_max,
+                void *stream);
+
+/* C = beta*C + alpha * X(x_row0 : x_row0+m, :) * Y(y_row0 : y_row0+n, :)^T
+ * with packed FP16 panels X, Y of k columns (k % 64 == 0), FP32
+ * accumulation, C column-major FP32 (ldc >= m).  x_row0 and y_row0 are
+ * multiples of 256.  beta = 0 does not read C.  Replaces
+ * GPUBLAS_SGEMM_EX -> rocblas_gemm_ex(f16, f16 -> f32, compute f32)
+ * (rocm_device_macros.h:267-268) at the Schur update
+ * (panel_trf_gdirect.hpp:791-795, :836-840, :868-872, :905-909) and the
+ * -alt panel products (:158-162, :178-182). */
+int omx_gemm_f16_packed(int64_t m, int64_t n, int64_t k, float alpha,
+                        const uint16_t *X, int64_t x_row0, const uint16_t *Y,
+                        int64_t y_row0, float beta, float *C, int64_t ldc,
+                        void *stream);
+
+/* As omx_gemm_f16_packed with FP8 e4m3 panels and their tile exponents
+ * (k % 256 == 0); the exponents enter the MFMA as E8M0 block scales, so each
+ * product is exact and only the FP32 accumulation rounds.  No reference
+ * counterpart (FP8 is new). */
+int omx_gemm_f8_packed(int64_t m, int64_t n, int64_t k, float alpha,
+                       const uint8_t *X, const int32_t *xexp, int64_t x_row0,
+                       const uint8_t *Y, const int32_t *yexp, int64_t y_row0,
+                       float beta, float *C, int64_t ldc, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * FP32 dense kernels of the panel factorisation.
+ * ------------------------------------------------------------------------- */
+
+/* C = beta*C + alpha * A * B, FP32, no transposes (k % 16 == 0). */
+int omx_sgemm_f32(int64_t m, int64_t n, int64_t k, float alpha,
+                  const float *A, int64_t lda, const float *B, int64_t ldb,
+                  float beta, float *C, int64_t ldc, void *stream);
+
+/* In-place no-pivot LU of the n x n block A (n % 64 == 0): unit-lower L
+ * below the diagonal, U on and above it.  *info (device, optional) is set to
+ * the 1-based index of the first zero pivot, or INT_MAX if none.  Replaces
+ * GPUSOLVER_SGETRF -> rocsolver_sgetrf_npvt (rocm_device_macros.h:290-291,
+ * panel_trf_gdirect.hpp:709-711, :800-801). */
+int omx_getrf_nopiv_f32(int64_t n, float *A, int64_t lda, int *info,
+                        void *stream);
+
+/* Triangular solve with multiple right-hand sides (variant OMX_TRSM_*),
+ * B m x n; the triangle's order (n for RUNN/RLNU, m for LLNU) is a
+ * multiple of 64.  Replaces GPUBLAS_STRSM -> rocblas_strsm
+ * (rocm_device_macros.h:248-249) in update_left/right_panels_wGPU
+ * (panel_trf.hpp:188-246) and the -alt 1 inverses on the identity
+ * (panel_trf_gdirect.hpp:89-92, :122-125). */
+int omx_trsm_f32(int variant, int64_t m, int64_t n, const float *T,
+                 int64_t ldt, float *B, int64_t ldb, void *stream);
+
+/* A = I (n x n, n <= 65535).  Replaces gen_identity_mat
+ * (fp16_gpu_kernels.cpp:305-337). */
+int omx_identity_f32(int64_t n, float *A, int64_t lda, void *stream);
+
+/* B = A for an m x n block (n <= 65535).  Replaces the GPU_MEMCPY_2D of the
+ * diagonal block (panel_trf_gdirect.hpp:805-811). */
+int omx_copy_block_f32(int64_t m, int64_t n, const float *A, int64_t lda,
+                       float *B, int64_t ldb, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Solver: the blocked LU with look-ahead and the FP64 refinement, driven by
+ * the host.  A handle owns two HIP streams, events and the panel and vector
+ * workspaces (allocated once by omx_solver_create; the factorisation and
+ * refinement calls allocate nothing except GMRES's basis on first use of a
+ * larger restart).  Calls on one handle must not overlap.
+ * ------------------------------------------------------------------------- */
+typedef struct omx_solver omx_solver;
+
+#define OMX_SOLVER_FP8 1 /* flag of omx_solver_create: also allocate FP8 panels */
+
+/* Create a solver for order n, block b (n % b == 0, b % 256 == 0), panel
+ * method alt (0: FP32 TRSM, 1: FP16 GEMM with the TRSM-on-identity inverse,
+ * 2: as 1 with the triangular masks of the reference's -alt 2).  Replaces
+ * build_panels_wGPU / build_workspace (panel.hpp:177-187, :361-454). */
+int omx_solver_create(int64_t n, int64_t b, int alt, int flags,
+                      omx_solver **out);
+int omx_solver_destroy(omx_solver *s);
+
+/* In-place no-pivot LU of the order-n matrix A (device, lda >= n,
+ * lda % 4 == 0) in the step order of panel_lu_async_wGPU_gdirect_alt2
+ * (panel_trf_gdirect.hpp:593-1192) on one GPU, with the diagonal/panel work
+ * of step k overlapped with the trailing update of step k on a second
+ * stream.  sched (host, n/b ints, optional): 1 packs step k's Schur panels
+ * in scaled FP8 e4m3 (requires OMX_SOLVER_FP8), 0 in FP16.  Blocks until
+ * done.  timings (host, optional, 5 doubles): wall ms, summed ms of the
+ * trailing-update GEMMs (HIP events on their stream), their count, their
+ * flops, flops of the panel GEMMs.  Returns 0 or the 1-based index of the
+ * first zero pivot. */
+int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched,
+                  double *timings);
+
+/* FP64 refinement of A x = rhs with the FP32 factors LU (device) and the
+ * generated A (regenerated from seed; diag = its FP64 diagonal).
+ * method 0: classical IR (iterative_refinement2 GPU branch,
+ * iterative_refinement.hpp:141-171, x0 = rhs./diag, stop at harness < 16);
+ * method 1: GMRES-IR, the same outer loop with each correction solved by
+ * GMRES(restart) on (LU)^-1 A d = (LU)^-1 r to relative tolerance
+ * inner_tol.  Blocks until done.  out (host, 4 doubles): final residual,
+ * harness, corrections applied (-1 if not converged), total inner
+ * iterations.  trace (host, 2*maxit, optional): residual and harness per
+ * step.  Returns 0, or OMX_ERR_NOCONV. */
+int omx_ir_solve(omx_solver *s, const float *LU, int64_t lda, uint64_t seed,
+                 const double *diag, const double *rhs, double *x,
+                 int method, int maxit, int restart, double inner_tol,
+                 double *out, double *trace);
 
 /* hpl-harness ||r||/(||A|| ||x|| + ||b||)/(n * DBL_EPSILON/2) on the host
  * (iterative_refinement.hpp:154-157). */

@@ -36,6 +36,21 @@ SIGNATURES = {
     "omx_trsv_workspace_bytes": (sz, [i64]),
     "omx_trsv_lower_unit_f32f64": (i32, [i64, i64, vp, i64, vp, vp, sz, vp]),
     "omx_trsv_upper_f32f64": (i32, [i64, i64, vp, i64, vp, vp, sz, vp]),
+    "omx_panel_bytes": (sz, [i64, i64, i32]),
+    "omx_panel_exps_count": (sz, [i64, i64]),
+    "omx_pack_f16": (i32, [i64, i64, vp, i64, i64, i32, vp, vp, vp]),
+    "omx_pack_f8": (i32, [i64, i64, vp, i64, i64, vp, vp, vp, vp]),
+    "omx_gemm_f16_packed": (i32, [i64, i64, i64, flt, vp, i64, vp, i64, flt, vp, i64, vp]),
+    "omx_gemm_f8_packed": (i32, [i64, i64, i64, flt, vp, vp, i64, vp, vp, i64, flt, vp, i64, vp]),
+    "omx_sgemm_f32": (i32, [i64, i64, i64, flt, vp, i64, vp, i64, flt, vp, i64, vp]),
+    "omx_getrf_nopiv_f32": (i32, [i64, vp, i64, vp, vp]),
+    "omx_trsm_f32": (i32, [i32, i64, i64, vp, i64, vp, i64, vp]),
+    "omx_identity_f32": (i32, [i64, vp, i64, vp]),
+    "omx_copy_block_f32": (i32, [i64, i64, vp, i64, vp, i64, vp]),
+    "omx_solver_create": (i32, [i64, i64, i32, i32, C.POINTER(vp)]),
+    "omx_solver_destroy": (i32, [vp]),
+    "omx_lu_factor": (i32, [vp, vp, i64, vp, vp]),
+    "omx_ir_solve": (i32, [vp, vp, i64, u64, vp, vp, vp, i32, i32, i32, dbl, vp, vp]),
     "omx_harness": (dbl, [dbl, dbl, dbl, dbl, i64]),
 }
 

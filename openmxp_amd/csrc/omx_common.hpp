@@ -73,4 +73,5 @@ static inline hipStream_t omx_stream(void *s) { return reinterpret_cast<hipStrea
         if (e_ != hipSuccess) return OMX_ERR_HIP;           \
     } while (0)
 
-static inline int64_t omx_cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+__host__ __device__ __forceinline__ int64_t omx_cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+#define omx_cdiv_dev omx_cdiv

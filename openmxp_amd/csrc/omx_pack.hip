@@ -1,0 +1,220 @@
+// omx_pack.hip -- FP32 -> FP16 / scaled FP8 e4m3 panel packing into the GEMM
+// layouts of omx_layout.hpp, plus the per-tile max-abs statistics used for
+// the FP8 power-of-two scales and by the precision scheduler.
+//
+// A source element (r, k) of an R x K panel is src[r*rs + k*ls]: the left
+// (column) panel A[k+1:, k] has rs = 1, ls = lda; the right (row) panel
+// A[k, k+1:] read transposed has rs = lda, ls = 1 -- the two conversions of
+// half_conversion_left / half_conversion_right_trans
+// (fp16_gpu_kernels.cpp:11-36, :157-211).  Rows past R are packed as zeros.
+#include "omx_common.hpp"
+#include "omx_layout.hpp"
+
+namespace {
+
+constexpr int kPackThreads = 256;
+constexpr int kTileK = 64;  // columns staged through LDS per workgroup
+
+// mask on the packed coordinates (r, k) for the -alt 2 triangular copies
+// (downCast_copy_upper / _lower, fp16_gpu_kernels.cpp:39-94)
+__device__ __forceinline__ float apply_mask(float v, int64_t r, int64_t k, int mask) {
+    switch (mask) {
+        case OMX_MASK_K_LE_R: return (k <= r) ? v : 0.0f;
+        case OMX_MASK_K_GT_R_UNIT: return (k > r) ? v : (k == r ? 1.0f : 0.0f);
+        case OMX_MASK_R_GT_K_UNIT: return (r > k) ? v : (r == k ? 1.0f : 0.0f);
+        default: return v;
+    }
+}
+
+// Stage the 256 x 64 source tile (rows r0.., columns k0..) in LDS as
+// tile[k][r]; loads are coalesced along whichever dimension is contiguous.
+__device__ __forceinline__ void load_tile(float (*tile)[OMX_STRIP + 1], const float *__restrict__ src,
+                                          int64_t rs, int64_t ls, int64_t R, int64_t r0, int64_t k0,
+                                          int mask) {
+    const int t = threadIdx.x;
+    if (rs == 1) {
+        const int64_t r = r0 + t;
+#pragma unroll 8
+        for (int k = 0; k < kTileK; ++k) {
+            float v = (r < R) ? src[r + (k0 + k) * ls] : 0.0f;
+            tile[k][t] = (r < R) ? apply_mask(v, r, k0 + k, mask) : 0.0f;
+        }
+    } else {
+        // contiguous along k: 16 float4 per row, 4 rows per wave per step
+#pragma unroll 4
+        for (int q = 0; q < 16; ++q) {
+            const int idx = t + kPackThreads * q;
+            const int rl = idx >> 4, k4 = (idx & 15) * 4;
+            const int64_t r = r0 + rl;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r < R) v = *reinterpret_cast<const float4 *>(src + r * rs + (k0 + k4));
+            tile[k4 + 0][rl] = (r < R) ? apply_mask(v.x, r, k0 + k4 + 0, mask) : 0.0f;
+            tile[k4 + 1][rl] = (r < R) ? apply_mask(v.y, r, k0 + k4 + 1, mask) : 0.0f;
+            tile[k4 + 2][rl] = (r < R) ? apply_mask(v.z, r, k0 + k4 + 2, mask) : 0.0f;
+            tile[k4 + 3][rl] = (r < R) ? apply_mask(v.w, r, k0 + k4 + 3, mask) : 0.0f;
+        }
+    }
+}
+
+// One workgroup per (row strip, 64-column stage): 32 KB of packed output.
+__global__ void __launch_bounds__(kPackThreads)
+k_pack_f16(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, int64_t ls,
+           int mask, _Float16 *__restrict__ P, float *__restrict__ chunk_max) {
+    __shared__ float tile[kTileK][OMX_STRIP + 1];
+    __shared__ float red[kPackThreads / 64];
+    const int64_t ks = blockIdx.x;   // K-stage
+    const int64_t rsb = blockIdx.y;  // row strip
+    const int64_t r0 = rsb * OMX_STRIP, k0 = ks * kTileK;
+    load_tile(tile, src, rs, ls, R, r0, k0, mask);
+    __syncthreads();
+    _Float16 *out = P + (rsb * (K >> 6) + ks) * 16384;
+    const int t = threadIdx.x;
+    float mx = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int oidx = t + kPackThreads * q;  // 16-byte unit, 0..2047
+        const int fb = oidx >> 6, lane = oidx & 63;
+        const int rl = (fb >> 2) * 32 + (lane & 31);
+        const int kb = (fb & 3) * 16 + (lane >> 5) * 8;
+        typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+        half8 h;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float v = tile[kb + e][rl];
+            mx = fmaxf(mx, fabsf(v));
+            h[e] = (_Float16)v;
+        }
+        *reinterpret_cast<half8 *>(out + oidx * 8) = h;
+    }
+    if (chunk_max) {
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+        if ((t & 63) == 0) red[t >> 6] = mx;
+        __syncthreads();
+        if (t == 0)
+            chunk_max[rsb * (K >> 6) + ks] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    }
+}
+
+// Max |x| of each 256 x 256 (rows x K) scale tile.
+__global__ void __launch_bounds__(kPackThreads)
+k_tile_maxabs(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, int64_t ls,
+              float *__restrict__ tmax) {
+    __shared__ float red[kPackThreads / 64];
+    const int64_t kt = blockIdx.x, rsb = blockIdx.y;
+    const int64_t r0 = rsb * OMX_STRIP, k0 = kt * OMX_SCALE_TILE;
+    const int t = threadIdx.x;
+    float mx = 0.0f;
+    if (rs == 1) {
+        const int64_t r = r0 + t;
+        if (r < R)
+            for (int64_t k = k0; k < k0 + OMX_SCALE_TILE && k < K; ++k) mx = fmaxf(mx, fabsf(src[r + k * ls]));
+    } else {
+        for (int q = 0; q < OMX_STRIP; ++q) {
+            const int64_t r = r0 + q;
+            const int64_t k = k0 + t;
+            if (r < R && k < K) mx = fmaxf(mx, fabsf(src[r * rs + k]));
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((t & 63) == 0) red[t >> 6] = mx;
+    __syncthreads();
+    if (t == 0) tmax[rsb * omx_cdiv_dev(K, OMX_SCALE_TILE) + kt] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+// largest e with maxabs * 2^e <= 448, clamped to [-100, 100]; 0 for zero
+__device__ __forceinline__ int pow2_exp_448(float maxabs) {
+    if (!(maxabs > 0.0f) || !(maxabs < __builtin_inff())) return 0;
+    int em;
+    const float fm = frexpf(maxabs, &em);  // maxabs = fm * 2^em, fm in [0.5, 1)
+    int e = (fm <= 0.875f) ? (9 - em) : (8 - em);  // 448 = 0.875 * 2^9
+    return e > 100 ? 100 : (e < -100 ? -100 : e);
+}
+
+__global__ void k_tile_exps(int64_t n, const float *__restrict__ tmax, int32_t *__restrict__ exps) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) exps[i] = pow2_exp_448(tmax[i]);
+}
+
+// FP8 pack: one workgroup per (row strip, 64-column half of a 128 stage).
+__global__ void __launch_bounds__(kPackThreads)
+k_pack_f8(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, int64_t ls,
+          const int32_t *__restrict__ exps, uint8_t *__restrict__ P) {
+    __shared__ float tile[kTileK][OMX_STRIP + 1];
+    const int64_t k64 = blockIdx.x;  // 64-column block
+    const int64_t rsb = blockIdx.y;
+    const int64_t r0 = rsb * OMX_STRIP, k0 = k64 * kTileK;
+    load_tile(tile, src, rs, ls, R, r0, k0, OMX_MASK_NONE);
+    __syncthreads();
+    const int64_t KT = omx_cdiv_dev(K, OMX_SCALE_TILE);
+    const float scale = ldexpf(1.0f, exps[rsb * KT + k0 / OMX_SCALE_TILE]);
+    const int64_t ks = k64 >> 1;
+    const int kk = (int)(k64 & 1);
+    uint8_t *chunk = P + (rsb * (K >> 7) + ks) * 32768;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int oidx = t + kPackThreads * q;  // 0..1023: (rr, h, lane)
+        const int rr = oidx >> 7, h = (oidx >> 6) & 1, lane = oidx & 63;
+        const int rl = rr * 32 + (lane & 31);
+        const int kb = (lane >> 5) * 32 + h * 16;  // within the 64-column block
+        uint32_t w[4];
+#pragma unroll
+        for (int e4 = 0; e4 < 4; ++e4) {
+            int pk = 0;
+            pk = __builtin_amdgcn_cvt_pk_fp8_f32(tile[kb + 4 * e4 + 0][rl] * scale,
+                                                 tile[kb + 4 * e4 + 1][rl] * scale, pk, false);
+            pk = __builtin_amdgcn_cvt_pk_fp8_f32(tile[kb + 4 * e4 + 2][rl] * scale,
+                                                 tile[kb + 4 * e4 + 3][rl] * scale, pk, true);
+            w[e4] = (uint32_t)pk;
+        }
+        uint8_t *dst = chunk + ((rr * 2 + kk) * 2048) + h * 1024 + lane * 16;
+        *reinterpret_cast<uint4 *>(dst) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t omx_panel_bytes(int64_t rows, int64_t k, int fp8) {
+    if (rows <= 0 || k <= 0) return 0;
+    return (size_t)omx_strips(rows) * OMX_STRIP * (size_t)k * (fp8 ? 1 : 2);
+}
+
+size_t omx_panel_exps_count(int64_t rows, int64_t k) {
+    if (rows <= 0 || k <= 0) return 0;
+    return (size_t)omx_strips(rows) * (size_t)omx_cdiv(k, OMX_SCALE_TILE);
+}
+
+int omx_pack_f16(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t ls, int mask,
+                 uint16_t *P, float *chunk_max, void *stream) {
+    if (rows <= 0 || k <= 0 || k % OMX_KSTAGE16 || !src || !P || mask < 0 || mask > 3) return OMX_ERR_ARG;
+    if (!((rs == 1 && ls >= rows) || (ls == 1 && rs >= k))) return OMX_ERR_ARG;
+    if (ls == 1 && (rs % 4 || (reinterpret_cast<uintptr_t>(src) & 15))) return OMX_ERR_ARG;
+    dim3 grid((unsigned)(k / kTileK), (unsigned)omx_strips(rows));
+    hipLaunchKernelGGL(k_pack_f16, grid, dim3(kPackThreads), 0, omx_stream(stream), rows, k, src, rs,
+                       ls, mask, reinterpret_cast<_Float16 *>(P), chunk_max);
+    OMX_LAUNCH_CHECK();
+    return OMX_OK;
+}
+
+int omx_pack_f8(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t ls, uint8_t *P,
+                int32_t *exps, float *tile_max, void *stream) {
+    if (rows <= 0 || k <= 0 || k % OMX_SCALE_TILE || !src || !P || !exps || !tile_max) return OMX_ERR_ARG;
+    if (!((rs == 1 && ls >= rows) || (ls == 1 && rs >= k))) return OMX_ERR_ARG;
+    if (ls == 1 && (rs % 4 || (reinterpret_cast<uintptr_t>(src) & 15))) return OMX_ERR_ARG;
+    hipStream_t st = omx_stream(stream);
+    const int64_t strips = omx_strips(rows), KT = omx_cdiv(k, OMX_SCALE_TILE);
+    hipLaunchKernelGGL(k_tile_maxabs, dim3((unsigned)KT, (unsigned)strips), dim3(kPackThreads), 0, st,
+                       rows, k, src, rs, ls, tile_max);
+    OMX_LAUNCH_CHECK();
+    const int64_t nt = strips * KT;
+    hipLaunchKernelGGL(k_tile_exps, dim3((unsigned)omx_cdiv(nt, 256)), dim3(256), 0, st, nt, tile_max, exps);
+    OMX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pack_f8, dim3((unsigned)(k / kTileK), (unsigned)strips), dim3(kPackThreads), 0,
+                       st, rows, k, src, rs, ls, exps, P);
+    OMX_LAUNCH_CHECK();
+    return OMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,516 @@
+// omx_solver.cpp -- host driver of the solver: blocked no-pivot LU with
+// look-ahead on two HIP streams, and FP64 iterative refinement (classical
+// and GMRES) on the GPU.
+//
+// LU (panel_lu_async_wGPU_gdirect_alt2, panel_trf_gdirect.hpp:593-1192, on a
+// 1x1 grid).  Step k (1 <= k < nb) of the reference does, with the panels
+// L_{k-1}, R_{k-1} of the previous step:
+//   (a) A_kk -= L_{k-1}[k] R_{k-1}[k]^T; getrf(A_kk)
+//   (c) A[k+1:,k] -= L_{k-1}[k+1:] R_{k-1}[k]^T; column panel by -alt; L_k
+//   (d) A[k,k+1:] -= L_{k-1}[k] R_{k-1}[k+1:]^T; row panel by -alt; R_k
+//   (e) A[k+1:,k+1:] -= L_{k-1}[k+1:] R_{k-1}[k+1:]^T      (the bulk)
+// Here (a)-(d) run on the critical stream and (e) on the bulk stream.
+// crit(k) waits for bulk(k-1) (its blocks were in bulk(k-1)'s region) and
+// bulk(k) waits for crit(k-1) (it needs L_{k-1}), so crit(k) overlaps
+// bulk(k): the diagonal factorisation and the panels of step k hide behind
+// the trailing update, which the reference serialises on the NULL stream.
+// Every block still receives the updates of panels 0, 1, 2, ... in order,
+// each as one GEMM accumulated into FP32 storage, as in the reference.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "omx_common.hpp"
+#include "omx_layout.hpp"
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+int dalloc(DevBuf &b, size_t bytes) {
+    b.bytes = bytes;
+    if (bytes == 0) return OMX_OK;
+    return hipMalloc(&b.p, bytes) == hipSuccess ? OMX_OK : OMX_ERR_HIP;
+}
+
+void dfree(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+}
+
+}  // namespace
+
+struct omx_solver {
+    int64_t n = 0, b = 0, nb = 0;
+    int alt = 1;
+    int fp8 = 0;
+    hipStream_t crit = nullptr, bulk = nullptr;
+    std::vector<hipEvent_t> ev_panel, ev_bulk, ev_t0, ev_t1;
+    DevBuf Lp[2], Rp[2], Tp;            // FP16 panels and the -alt temporary
+    DevBuf L8[2], R8[2], Le[2], Re[2];  // FP8 panels and exponents
+    DevBuf L8max, R8max;                // tile max-abs scratch
+    DevBuf Wup, Wlo, Wup_h, Wlo_h;      // -alt inverses (FP32, packed FP16)
+    DevBuf cmaxL, cmaxR;                // chunk max-abs of the FP16 packs
+    DevBuf info;
+    // refinement
+    DevBuf r, w, gemv_ws, trsv_ws, nrm, V, hv, dots;
+    int restart_cap = 0;
+    // timings of the last factorisation
+    double t_lu_ms = 0, bulk_ms = 0, bulk_flops = 0;
+    int bulk_count = 0;
+    double crit_flops = 0;
+};
+
+namespace {
+
+constexpr int kDotBlocks = 512;
+
+__global__ void k_dot_partial(int64_t n, int nv, const double *__restrict__ V, int64_t ldv,
+                              const double *__restrict__ w, double *__restrict__ part) {
+    // part[v * gridDim.x + block] = sum over this block's slice of V[:,v] * w
+    __shared__ double sh[256];
+    const int64_t chunk = omx_cdiv(n, gridDim.x);
+    const int64_t i0 = (int64_t)blockIdx.x * chunk;
+    const int64_t i1 = (i0 + chunk < n) ? i0 + chunk : n;
+    for (int v = 0; v < nv; ++v) {
+        const double *col = V + (int64_t)v * ldv;
+        double s = 0.0;
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) s += col[i] * w[i];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(int64_t)v * gridDim.x + blockIdx.x] = sh[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_dot_final(int nv, int nparts, const double *__restrict__ part, double *__restrict__ out) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double s = 0.0;
+    for (int q = 0; q < nparts; ++q) s += part[(int64_t)v * nparts + q];
+    out[v] = s;
+}
+
+// w -= V[:, 0:nv] * h  (h on the device)
+__global__ void k_multi_axpy(int64_t n, int nv, const double *__restrict__ V, int64_t ldv,
+                             const double *__restrict__ h, double *__restrict__ w) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = w[i];
+        for (int v = 0; v < nv; ++v) s -= V[i + (int64_t)v * ldv] * h[v];
+        w[i] = s;
+    }
+}
+
+// x += V[:, 0:nv] * y
+__global__ void k_multi_axpy_add(int64_t n, int nv, const double *__restrict__ V, int64_t ldv,
+                                 const double *__restrict__ y, double *__restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int v = 0; v < nv; ++v) s += V[i + (int64_t)v * ldv] * y[v];
+        x[i] += s;
+    }
+}
+
+__global__ void k_scale_copy(int64_t n, double a, const double *__restrict__ x, double *__restrict__ y) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = a * x[i];
+}
+
+__global__ void k_zero_d(int64_t n, double *__restrict__ y) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = 0.0;
+}
+
+unsigned vgrid(int64_t n) {
+    int64_t g = omx_cdiv(n, 256);
+    return (unsigned)(g > 16384 ? 16384 : (g < 1 ? 1 : g));
+}
+
+int dots(omx_solver *s, int64_t n, int nv, const double *V, int64_t ldv, const double *w, double *out,
+         hipStream_t st) {
+    double *part = static_cast<double *>(s->dots.p);
+    hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(256), 0, st, n, nv, V, ldv, w, part);
+    OMX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dot_final, dim3((unsigned)omx_cdiv(nv, 64)), dim3(64), 0, st, nv, kDotBlocks, part, out);
+    OMX_LAUNCH_CHECK();
+    return OMX_OK;
+}
+
+double wall_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+#define RC(x)                  \
+    do {                       \
+        int rc_ = (x);         \
+        if (rc_) return rc_;   \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out) {
+    if (!out || n <= 0 || b <= 0 || n % b || b % OMX_STRIP || alt < 0 || alt > 2) return OMX_ERR_ARG;
+    omx_solver *s = new omx_solver();
+    s->n = n;
+    s->b = b;
+    s->nb = n / b;
+    s->alt = alt;
+    s->fp8 = (flags & OMX_SOLVER_FP8) ? 1 : 0;
+    int rc = OMX_OK;
+    auto fail = [&](int code) {
+        omx_solver_destroy(s);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&s->crit, hipStreamNonBlocking) != hipSuccess) return fail(OMX_ERR_HIP);
+    if (hipStreamCreateWithFlags(&s->bulk, hipStreamNonBlocking) != hipSuccess) return fail(OMX_ERR_HIP);
+    const int64_t nb = s->nb;
+    s->ev_panel.resize(nb);
+    s->ev_bulk.resize(nb);
+    s->ev_t0.resize(nb);
+    s->ev_t1.resize(nb);
+    for (int64_t k = 0; k < nb; ++k) {
+        if (hipEventCreateWithFlags(&s->ev_panel[k], hipEventDisableTiming) != hipSuccess) return fail(OMX_ERR_HIP);
+        if (hipEventCreateWithFlags(&s->ev_bulk[k], hipEventDisableTiming) != hipSuccess) return fail(OMX_ERR_HIP);
+        if (hipEventCreate(&s->ev_t0[k]) != hipSuccess) return fail(OMX_ERR_HIP);
+        if (hipEventCreate(&s->ev_t1[k]) != hipSuccess) return fail(OMX_ERR_HIP);
+    }
+    const int64_t rows = (n > b) ? n - b : b;
+    const size_t pb16 = omx_panel_bytes(rows, b, 0);
+    for (int i = 0; i < 2; ++i) {
+        if ((rc = dalloc(s->Lp[i], pb16)) || (rc = dalloc(s->Rp[i], pb16))) return fail(rc);
+        if (s->fp8) {
+            const size_t pb8 = omx_panel_bytes(rows, b, 1);
+            const size_t ne = omx_panel_exps_count(rows, b);
+            if ((rc = dalloc(s->L8[i], pb8)) || (rc = dalloc(s->R8[i], pb8)) ||
+                (rc = dalloc(s->Le[i], ne * 4)) || (rc = dalloc(s->Re[i], ne * 4)))
+                return fail(rc);
+        }
+    }
+    if (s->fp8) {
+        const size_t ne = omx_panel_exps_count(rows, b);
+        if ((rc = dalloc(s->L8max, ne * 4)) || (rc = dalloc(s->R8max, ne * 4))) return fail(rc);
+    }
+    if ((rc = dalloc(s->Tp, pb16))) return fail(rc);
+    if ((rc = dalloc(s->Wup, (size_t)b * b * 4)) || (rc = dalloc(s->Wlo, (size_t)b * b * 4)) ||
+        (rc = dalloc(s->Wup_h, omx_panel_bytes(b, b, 0))) || (rc = dalloc(s->Wlo_h, omx_panel_bytes(b, b, 0))))
+        return fail(rc);
+    const size_t ncm = (size_t)omx_strips(rows) * (size_t)(b / OMX_KSTAGE16);
+    if ((rc = dalloc(s->cmaxL, ncm * 4)) || (rc = dalloc(s->cmaxR, ncm * 4))) return fail(rc);
+    if ((rc = dalloc(s->info, 16))) return fail(rc);
+    // refinement vectors
+    if ((rc = dalloc(s->r, n * 8)) || (rc = dalloc(s->w, n * 8)) ||
+        (rc = dalloc(s->gemv_ws, std::max<size_t>(omx_otf_gemv_workspace_bytes(n), 8))) ||
+        (rc = dalloc(s->trsv_ws, omx_trsv_workspace_bytes(n))) || (rc = dalloc(s->nrm, 64)))
+        return fail(rc);
+    *out = s;
+    return OMX_OK;
+}
+
+int omx_solver_destroy(omx_solver *s) {
+    if (!s) return OMX_OK;
+    if (s->crit) (void)hipStreamSynchronize(s->crit);
+    if (s->bulk) (void)hipStreamSynchronize(s->bulk);
+    for (auto *v : {&s->ev_panel, &s->ev_bulk, &s->ev_t0, &s->ev_t1})
+        for (auto e : *v)
+            if (e) (void)hipEventDestroy(e);
+    DevBuf *bufs[] = {&s->Lp[0], &s->Lp[1], &s->Rp[0], &s->Rp[1], &s->Tp, &s->L8[0], &s->L8[1], &s->R8[0],
+                      &s->R8[1], &s->Le[0], &s->Le[1], &s->Re[0], &s->Re[1], &s->L8max, &s->R8max,
+                      &s->Wup, &s->Wlo, &s->Wup_h, &s->Wlo_h, &s->cmaxL, &s->cmaxR, &s->info, &s->r,
+                      &s->w, &s->gemv_ws, &s->trsv_ws, &s->nrm, &s->V, &s->hv, &s->dots};
+    for (DevBuf *d : bufs) dfree(*d);
+    if (s->crit) (void)hipStreamDestroy(s->crit);
+    if (s->bulk) (void)hipStreamDestroy(s->bulk);
+    delete s;
+    return OMX_OK;
+}
+
+// Schur product C -= Lpanel(rows lr0..) * Rpanel(rows rr0..)^T in the
+// precision the panels of that step were packed in.
+static int schur(omx_solver *s, int use8, int cur, int64_t m, int64_t nn, int64_t lr0, int64_t rr0, float *C,
+                 int64_t ldc, hipStream_t st) {
+    const int64_t b = s->b;
+    if (use8)
+        return omx_gemm_f8_packed(m, nn, b, -1.0f, static_cast<uint8_t *>(s->L8[cur].p),
+                                  static_cast<int32_t *>(s->Le[cur].p), lr0, static_cast<uint8_t *>(s->R8[cur].p),
+                                  static_cast<int32_t *>(s->Re[cur].p), rr0, 1.0f, C, ldc, st);
+    return omx_gemm_f16_packed(m, nn, b, -1.0f, static_cast<uint16_t *>(s->Lp[cur].p), lr0,
+                               static_cast<uint16_t *>(s->Rp[cur].p), rr0, 1.0f, C, ldc, st);
+}
+
+// Pack the Schur panels of step k from the finished FP32 column / row panels.
+static int pack_panels(omx_solver *s, int nxt, int use8, int64_t rest, const float *Acol, const float *Arow,
+                       int64_t lda, hipStream_t st) {
+    const int64_t b = s->b;
+    if (use8) {
+        RC(omx_pack_f8(rest, b, Acol, 1, lda, static_cast<uint8_t *>(s->L8[nxt].p),
+                       static_cast<int32_t *>(s->Le[nxt].p), static_cast<float *>(s->L8max.p), st));
+        RC(omx_pack_f8(rest, b, Arow, lda, 1, static_cast<uint8_t *>(s->R8[nxt].p),
+                       static_cast<int32_t *>(s->Re[nxt].p), static_cast<float *>(s->R8max.p), st));
+        return OMX_OK;
+    }
+    RC(omx_pack_f16(rest, b, Acol, 1, lda, OMX_MASK_NONE, static_cast<uint16_t *>(s->Lp[nxt].p),
+                    static_cast<float *>(s->cmaxL.p), st));
+    RC(omx_pack_f16(rest, b, Arow, lda, 1, OMX_MASK_NONE, static_cast<uint16_t *>(s->Rp[nxt].p),
+                    static_cast<float *>(s->cmaxR.p), st));
+    return OMX_OK;
+}
+
+int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double *timings) {
+    if (!s || !A || lda < s->n || lda % 4) return OMX_ERR_ARG;
+    const int64_t n = s->n, b = s->b, nb = s->nb;
+    hipStream_t cs = s->crit, bs = s->bulk;
+    if (sched && !s->fp8)
+        for (int64_t k = 0; k < nb; ++k)
+            if (sched[k]) return OMX_ERR_ARG;  // FP8 buffers were not allocated
+    int *info = static_cast<int *>(s->info.p);
+    uint16_t *Tp = static_cast<uint16_t *>(s->Tp.p);
+    float *Wup = static_cast<float *>(s->Wup.p), *Wlo = static_cast<float *>(s->Wlo.p);
+    uint16_t *Wup_h = static_cast<uint16_t *>(s->Wup_h.p), *Wlo_h = static_cast<uint16_t *>(s->Wlo_h.p);
+    s->bulk_count = 0;
+    s->bulk_flops = 0;
+    s->crit_flops = 0;
+    OMX_HIP_CHECK(hipDeviceSynchronize());
+    const double t0 = wall_ms();
+
+    // step 0: getrf(A00) and the two FP32 TRSM panels
+    // (panel_trf_gdirect.hpp:699-737; panel_trf.hpp:188-246)
+    RC(omx_getrf_nopiv_f32(b, A, lda, info, cs));
+    int use8_prev = (sched && sched[0]) ? 1 : 0;
+    if (nb > 1) {
+        const int64_t rest = n - b;
+        RC(omx_trsm_f32(OMX_TRSM_RUNN, rest, b, A, lda, A + b, lda, cs));
+        RC(omx_trsm_f32(OMX_TRSM_LLNU, b, rest, A, lda, A + b * lda, lda, cs));
+        RC(pack_panels(s, 0, use8_prev, rest, A + b, A + b * lda, lda, cs));
+    }
+    OMX_HIP_CHECK(hipEventRecord(s->ev_panel[0], cs));
+
+    for (int64_t k = 1; k < nb; ++k) {
+        const int cur = (int)((k - 1) & 1), nxt = (int)(k & 1);
+        const int64_t kb = k * b;
+        const int64_t rest = n - kb - b;
+        float *Akk = A + kb + kb * lda;
+        const int use8 = (sched && sched[k]) ? 1 : 0;
+        // ---- critical path: diagonal block, column and row panels ----
+        if (k >= 2) OMX_HIP_CHECK(hipStreamWaitEvent(cs, s->ev_bulk[k - 1], 0));
+        RC(schur(s, use8_prev, cur, b, b, 0, 0, Akk, lda, cs));
+        RC(omx_getrf_nopiv_f32(b, Akk, lda, info, cs));
+        if (rest > 0) {
+            float *Acol = A + (kb + b) + kb * lda;
+            float *Arow = A + kb + (kb + b) * lda;
+            // column panel (:832-860)
+            RC(schur(s, use8_prev, cur, rest, b, b, 0, Acol, lda, cs));
+            if (s->alt == 0) {
+                RC(omx_trsm_f32(OMX_TRSM_RUNN, rest, b, Akk, lda, Acol, lda, cs));
+            } else {
+                RC(omx_identity_f32(b, Wup, b, cs));
+                RC(omx_trsm_f32(OMX_TRSM_RUNN, b, b, Akk, lda, Wup, b, cs));
+                // Y(j, l) = W(l, j), upper part only for -alt 2
+                RC(omx_pack_f16(b, b, Wup, b, 1, s->alt == 2 ? OMX_MASK_K_LE_R : OMX_MASK_NONE, Wup_h, nullptr, cs));
+                RC(omx_pack_f16(rest, b, Acol, 1, lda, OMX_MASK_NONE, Tp, nullptr, cs));
+                RC(omx_gemm_f16_packed(rest, b, b, 1.0f, Tp, 0, Wup_h, 0, 0.0f, Acol, lda, cs));
+            }
+            // row panel (:864-892)
+            RC(schur(s, use8_prev, cur, b, rest, 0, b, Arow, lda, cs));
+            if (s->alt == 0) {
+                RC(omx_trsm_f32(OMX_TRSM_LLNU, b, rest, Akk, lda, Arow, lda, cs));
+            } else {
+                RC(omx_identity_f32(b, Wlo, b, cs));
+                RC(omx_trsm_f32(OMX_TRSM_RLNU, b, b, Akk, lda, Wlo, b, cs));
+                // X(i, l) = W(i, l), strict lower + unit diagonal for -alt 2
+                RC(omx_pack_f16(b, b, Wlo, 1, b, s->alt == 2 ? OMX_MASK_R_GT_K_UNIT : OMX_MASK_NONE, Wlo_h, nullptr, cs));
+                RC(omx_pack_f16(rest, b, Arow, lda, 1, OMX_MASK_NONE, Tp, nullptr, cs));
+                RC(omx_gemm_f16_packed(b, rest, b, 1.0f, Wlo_h, 0, Tp, 0, 0.0f, Arow, lda, cs));
+            }
+            RC(pack_panels(s, nxt, use8, rest, Acol, Arow, lda, cs));
+            s->crit_flops += 4.0 * rest * (double)b * b;
+        }
+        OMX_HIP_CHECK(hipEventRecord(s->ev_panel[k], cs));
+        // ---- bulk: trailing update with the previous panels (:900-915) ----
+        if (rest > 0) {
+            OMX_HIP_CHECK(hipStreamWaitEvent(bs, s->ev_panel[k - 1], 0));
+            OMX_HIP_CHECK(hipEventRecord(s->ev_t0[s->bulk_count], bs));
+            RC(schur(s, use8_prev, cur, rest, rest, b, b, A + (kb + b) + (kb + b) * lda, lda, bs));
+            OMX_HIP_CHECK(hipEventRecord(s->ev_t1[s->bulk_count], bs));
+            s->bulk_flops += 2.0 * (double)rest * rest * b;
+            ++s->bulk_count;
+        }
+        OMX_HIP_CHECK(hipEventRecord(s->ev_bulk[k], bs));
+        use8_prev = use8;
+    }
+    OMX_HIP_CHECK(hipStreamSynchronize(cs));
+    OMX_HIP_CHECK(hipStreamSynchronize(bs));
+    s->t_lu_ms = wall_ms() - t0;
+    s->bulk_ms = 0;
+    for (int i = 0; i < s->bulk_count; ++i) {
+        float ms = 0;
+        OMX_HIP_CHECK(hipEventElapsedTime(&ms, s->ev_t0[i], s->ev_t1[i]));
+        s->bulk_ms += ms;
+    }
+    int hinfo = 0;
+    OMX_HIP_CHECK(hipMemcpy(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost));
+    if (timings) {
+        timings[0] = s->t_lu_ms;
+        timings[1] = s->bulk_ms;
+        timings[2] = s->bulk_count;
+        timings[3] = s->bulk_flops;
+        timings[4] = s->crit_flops;
+    }
+    return (hinfo == 0x7fffffff) ? OMX_OK : hinfo;
+}
+
+// Refinement.  method 0: classical IR (iterative_refinement.hpp:141-171);
+// method 1: GMRES-IR (same outer loop; the correction solves A d = r by
+// GMRES(restart) on (LU)^-1 A d = (LU)^-1 r with classical Gram-Schmidt
+// applied twice, all in FP64).  Returns OMX_OK when the harness fell below
+// 16, OMX_ERR_NOCONV otherwise.  out[0..3] = residual, harness, corrections,
+// total inner GMRES iterations; trace (2*maxit) = residual, harness per step.
+int omx_ir_solve(omx_solver *s, const float *LU, int64_t lda, uint64_t seed, const double *diag,
+                 const double *rhs, double *x, int method, int maxit, int restart, double inner_tol,
+                 double *out, double *trace) {
+    if (!s || !LU || !diag || !rhs || !x || maxit <= 0 || lda < s->n) return OMX_ERR_ARG;
+    if (method == 1 && (restart <= 0 || restart > 512)) return OMX_ERR_ARG;
+    const int64_t n = s->n, b = s->b;
+    hipStream_t st = s->crit;
+    double *r = static_cast<double *>(s->r.p), *w = static_cast<double *>(s->w.p);
+    double *nrm = static_cast<double *>(s->nrm.p);
+    if (method == 1 && s->restart_cap < restart) {
+        dfree(s->V);
+        dfree(s->hv);
+        dfree(s->dots);
+        RC(dalloc(s->V, (size_t)(restart + 1) * n * 8));
+        RC(dalloc(s->hv, (size_t)(restart + 2) * 8));
+        RC(dalloc(s->dots, (size_t)(restart + 2) * kDotBlocks * 8));
+        s->restart_cap = restart;
+    }
+    double hn[4];
+    RC(omx_norm_inf_f64(n, diag, nrm, st));
+    RC(omx_norm_inf_f64(n, rhs, nrm + 1, st));
+    OMX_HIP_CHECK(hipMemcpyAsync(hn, nrm, 16, hipMemcpyDeviceToHost, st));
+    OMX_HIP_CHECK(hipStreamSynchronize(st));
+    const double norma = 2.0 * hn[0], normb = hn[1];
+    RC(omx_div_f64(n, rhs, diag, x, st));
+    int inner_total = 0, result = OMX_ERR_NOCONV, it = 0;
+    double normr = -1, harness = -1;
+    for (it = 0; it < maxit; ++it) {
+        RC(omx_copy_f64(n, rhs, r, st));
+        RC(omx_norm_inf_f64(n, x, nrm, st));
+        RC(omx_otf_gemv_f64(n, seed, diag, -1.0, x, r, s->gemv_ws.p, s->gemv_ws.bytes, st));
+        RC(omx_norm_inf_f64(n, r, nrm + 1, st));
+        OMX_HIP_CHECK(hipMemcpyAsync(hn, nrm, 16, hipMemcpyDeviceToHost, st));
+        OMX_HIP_CHECK(hipStreamSynchronize(st));
+        const double normx = hn[0];
+        normr = hn[1];
+        harness = omx_harness(normr, norma, normx, normb, n);
+        if (trace) {
+            trace[2 * it] = normr;
+            trace[2 * it + 1] = harness;
+        }
+        if (harness < 16.0) {
+            result = OMX_OK;
+            break;
+        }
+        if (!std::isfinite(harness)) break;
+        if (method == 0) {
+            RC(omx_trsv_lower_unit_f32f64(n, b, LU, lda, r, s->trsv_ws.p, s->trsv_ws.bytes, st));
+            RC(omx_trsv_upper_f32f64(n, b, LU, lda, r, s->trsv_ws.p, s->trsv_ws.bytes, st));
+            RC(omx_axpy_f64(n, 1.0, r, x, st));
+            continue;
+        }
+        // ---- GMRES(restart) on the preconditioned correction equation ----
+        const int m = restart;
+        double *V = static_cast<double *>(s->V.p), *hv = static_cast<double *>(s->hv.p);
+        RC(omx_trsv_lower_unit_f32f64(n, b, LU, lda, r, s->trsv_ws.p, s->trsv_ws.bytes, st));
+        RC(omx_trsv_upper_f32f64(n, b, LU, lda, r, s->trsv_ws.p, s->trsv_ws.bytes, st));
+        RC(dots(s, n, 1, r, n, r, hv, st));
+        double beta = 0;
+        OMX_HIP_CHECK(hipMemcpyAsync(&beta, hv, 8, hipMemcpyDeviceToHost, st));
+        OMX_HIP_CHECK(hipStreamSynchronize(st));
+        beta = std::sqrt(beta);
+        if (!(beta > 0) || !std::isfinite(beta)) break;
+        hipLaunchKernelGGL(k_scale_copy, dim3(vgrid(n)), dim3(256), 0, st, n, 1.0 / beta, r, V);
+        OMX_LAUNCH_CHECK();
+        std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), y(m), h(m + 2);
+        g[0] = beta;
+        int jdone = 0;
+        for (int j = 0; j < m; ++j) {
+            double *vj = V + (int64_t)j * n;
+            hipLaunchKernelGGL(k_zero_d, dim3(vgrid(n)), dim3(256), 0, st, n, w);
+            OMX_LAUNCH_CHECK();
+            RC(omx_otf_gemv_f64(n, seed, diag, 1.0, vj, w, s->gemv_ws.p, s->gemv_ws.bytes, st));
+            RC(omx_trsv_lower_unit_f32f64(n, b, LU, lda, w, s->trsv_ws.p, s->trsv_ws.bytes, st));
+            RC(omx_trsv_upper_f32f64(n, b, LU, lda, w, s->trsv_ws.p, s->trsv_ws.bytes, st));
+            // classical Gram-Schmidt, twice
+            std::fill(h.begin(), h.end(), 0.0);
+            for (int pass = 0; pass < 2; ++pass) {
+                RC(dots(s, n, j + 1, V, n, w, hv, st));
+                hipLaunchKernelGGL(k_multi_axpy, dim3(vgrid(n)), dim3(256), 0, st, n, j + 1, V, n, hv, w);
+                OMX_LAUNCH_CHECK();
+                std::vector<double> hp(j + 1);
+                OMX_HIP_CHECK(hipMemcpyAsync(hp.data(), hv, 8 * (j + 1), hipMemcpyDeviceToHost, st));
+                OMX_HIP_CHECK(hipStreamSynchronize(st));
+                for (int i = 0; i <= j; ++i) h[i] += hp[i];
+            }
+            RC(dots(s, n, 1, w, n, w, hv, st));
+            double hn2 = 0;
+            OMX_HIP_CHECK(hipMemcpyAsync(&hn2, hv, 8, hipMemcpyDeviceToHost, st));
+            OMX_HIP_CHECK(hipStreamSynchronize(st));
+            const double hnext = std::sqrt(hn2);
+            if (hnext > 0) {
+                hipLaunchKernelGGL(k_scale_copy, dim3(vgrid(n)), dim3(256), 0, st, n, 1.0 / hnext, w,
+                                   V + (int64_t)(j + 1) * n);
+                OMX_LAUNCH_CHECK();
+            }
+            for (int i = 0; i <= j; ++i) H[i + (size_t)j * (m + 1)] = h[i];
+            H[(j + 1) + (size_t)j * (m + 1)] = hnext;
+            for (int i = 0; i < j; ++i) {
+                const double a = H[i + (size_t)j * (m + 1)], bb = H[(i + 1) + (size_t)j * (m + 1)];
+                H[i + (size_t)j * (m + 1)] = cs[i] * a + sn[i] * bb;
+                H[(i + 1) + (size_t)j * (m + 1)] = -sn[i] * a + cs[i] * bb;
+            }
+            const double a = H[j + (size_t)j * (m + 1)], bb = H[(j + 1) + (size_t)j * (m + 1)];
+            const double den = std::hypot(a, bb);
+            cs[j] = den > 0 ? a / den : 1.0;
+            sn[j] = den > 0 ? bb / den : 0.0;
+            H[j + (size_t)j * (m + 1)] = den;
+            H[(j + 1) + (size_t)j * (m + 1)] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            jdone = j + 1;
+            ++inner_total;
+            if (std::fabs(g[j + 1]) <= inner_tol * beta || hnext == 0.0) break;
+        }
+        for (int i = jdone - 1; i >= 0; --i) {
+            double sum = g[i];
+            for (int q = i + 1; q < jdone; ++q) sum -= H[i + (size_t)q * (m + 1)] * y[q];
+            y[i] = sum / H[i + (size_t)i * (m + 1)];
+        }
+        OMX_HIP_CHECK(hipMemcpyAsync(hv, y.data(), 8 * jdone, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_multi_axpy_add, dim3(vgrid(n)), dim3(256), 0, st, n, jdone, V, n, hv, x);
+        OMX_LAUNCH_CHECK();
+    }
+    OMX_HIP_CHECK(hipStreamSynchronize(st));
+    if (out) {
+        out[0] = normr;
+        out[1] = harness;
+        out[2] = (result == OMX_OK) ? it : -1;
+        out[3] = inner_total;
+    }
+    return result;
+}
+
+}  // extern "C"

@@ -1,0 +1,180 @@
+// omxp_main.cpp -- command-line driver with the reference's surface
+// (main.cpp:85-252, :273-736): positional N B [P Q] and the flags -solv,
+// -alt, -log, -progress, -init, plus -prec, -ir, -seed, -maxit, -restart,
+// -device.  One GPU, 1x1 grid.  The timed region is LU + IR, generation
+// excluded (main.cpp:614-657); the rate is (2/3 n^3 + 3/2 n^2) / time
+// (main.cpp:663).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include "omxp.h"
+
+namespace {
+
+struct Opts {
+    long n = 61440, b = 2560, p = 1, q = 1;
+    int solv = 1, alt = 1, log = 1, progress = 10, init = 1, skip = 0;
+    int ir = 0, maxit = 50, restart = 50, device = 0;
+    double inner_tol = 1e-10;
+    unsigned long long seed = 42;
+    std::string prec = "fp16";
+};
+
+void usage() {
+    std::printf(
+        "usage: omxp N B [P Q] [-solv 0|1] [-alt 0|1|2] [-log 0|1] [-progress k] [-init 1]\n"
+        "            [-prec fp16|fp8] [-ir classic|gmres] [-seed s] [-maxit k]\n"
+        "            [-restart m] [-device d]\n");
+}
+
+bool parse(int argc, char **argv, Opts &o) {
+    int pos = 0;
+    for (int i = 1; i < argc; ++i) {
+        const char *a = argv[i];
+        auto next = [&](const char *name) -> const char * {
+            if (i + 1 >= argc) {
+                std::fprintf(stderr, "missing value for %s\n", name);
+                std::exit(2);
+            }
+            return argv[++i];
+        };
+        if (a[0] != '-') {
+            long v = std::atol(a);
+            if (pos == 0) o.n = v;
+            else if (pos == 1) o.b = v;
+            else if (pos == 2) o.p = v;
+            else if (pos == 3) o.q = v;
+            ++pos;
+        } else if (!std::strcmp(a, "-solv")) o.solv = std::atoi(next(a));
+        else if (!std::strcmp(a, "-alt")) o.alt = std::atoi(next(a));
+        else if (!std::strcmp(a, "-log")) o.log = std::atoi(next(a));
+        else if (!std::strcmp(a, "-progress")) o.progress = std::atoi(next(a));
+        else if (!std::strcmp(a, "-init")) o.init = std::atoi(next(a));
+        else if (!std::strcmp(a, "-skip")) o.skip = std::atoi(next(a));
+        else if (!std::strcmp(a, "-prec")) o.prec = next(a);
+        else if (!std::strcmp(a, "-ir")) o.ir = !std::strcmp(next(a), "gmres") ? 1 : 0;
+        else if (!std::strcmp(a, "-seed")) o.seed = std::strtoull(next(a), nullptr, 10);
+        else if (!std::strcmp(a, "-maxit")) o.maxit = std::atoi(next(a));
+        else if (!std::strcmp(a, "-restart")) o.restart = std::atoi(next(a));
+        else if (!std::strcmp(a, "-device")) o.device = std::atoi(next(a));
+        else if (!std::strcmp(a, "-h") || !std::strcmp(a, "--help")) {
+            usage();
+            std::exit(0);
+        } else
+            std::printf("Ignoring unknown option: %s\n", a);
+    }
+    return true;
+}
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+#define CK(x)                                                                          \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) {                                                        \
+            std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+            return 1;                                                                  \
+        }                                                                              \
+    } while (0)
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    Opts o;
+    parse(argc, argv, o);
+    if (o.p != 1 || o.q != 1) {
+        std::fprintf(stderr, "omxp runs on one GPU: P x Q must be 1 x 1 (got %ld x %ld)\n", o.p, o.q);
+        return 2;
+    }
+    if (o.skip) {
+        std::fprintf(stderr, "-skip is not supported: every run does the whole LU and IR\n");
+        return 2;
+    }
+    if (o.init != 1) std::printf("\t-init %d: only the on-device generator (-init 1) exists; using it\n", o.init);
+    if (o.alt < 0 || o.alt > 2 || o.b <= 0 || o.b % 256) {
+        std::fprintf(stderr, "need -alt in 0..2 and B a multiple of 256\n");
+        return 2;
+    }
+    // N rounded to a multiple of B (main.cpp:341-351 with P = Q = 1)
+    if (o.n % o.b) {
+        long nf = (long)((float)o.n / (float)o.b + 0.5f);
+        if (nf == 0) nf = 1;
+        o.n = nf * o.b;
+    }
+    const bool fp8 = (o.prec == "fp8");
+    CK(hipSetDevice(o.device));
+    const long n = o.n, b = o.b, lda = n;
+    std::printf("===============================================================================\n");
+    std::printf("omxp - mixed-precision LU + iterative refinement on one MI355X (gfx950)\n");
+    std::printf("===============================================================================\n");
+    std::printf("\t%s\n", omx_version());
+    std::printf("\tn=%ld b=%ld 1x1 grid, alt=%d, Schur precision %s, IR %s (maxit %d)\n", n, b, o.alt,
+                fp8 ? "FP8 e4m3 (pow2 tile scale)" : "FP16", o.ir ? "GMRES" : "classical", o.maxit);
+
+    float *A = nullptr;
+    double *d = nullptr, *rhs = nullptr, *x = nullptr;
+    CK(hipMalloc(&A, sizeof(float) * (size_t)lda * n));
+    CK(hipMalloc(&d, sizeof(double) * n));
+    CK(hipMalloc(&rhs, sizeof(double) * n));
+    CK(hipMalloc(&x, sizeof(double) * n));
+    omx_solver *s = nullptr;
+    if (omx_solver_create(n, b, o.alt, fp8 ? OMX_SOLVER_FP8 : 0, &s) != 0) {
+        std::fprintf(stderr, "solver creation failed\n");
+        return 1;
+    }
+    double t = now_s();
+    if (omx_matgen_f32(n, o.seed, A, lda, d, nullptr) || omx_rhs_f64(n, o.seed, rhs, nullptr)) {
+        std::fprintf(stderr, "generation failed\n");
+        return 1;
+    }
+    CK(hipDeviceSynchronize());
+    std::printf("\tInitialization = %.4f sec\n", now_s() - t);
+
+    std::vector<int> sched(n / b, fp8 ? 1 : 0);
+    double tim[5] = {0};
+    const double start = now_s();
+    int info = omx_lu_factor(s, A, lda, fp8 ? sched.data() : nullptr, tim);
+    const double t_lu = now_s() - start;
+    std::printf("LU factorization %.2f sec (trailing GEMM %.2f sec over %d launches, %.1f TFLOP/s)\n", t_lu,
+                tim[1] * 1e-3, (int)tim[2], tim[1] > 0 ? tim[3] / (tim[1] * 1e-3) * 1e-12 : 0.0);
+    if (info) std::printf("LU reported a zero pivot at %d\n", info);
+    std::printf("\nIR Start with GPU\n");
+    const double start_ir = now_s();
+    double out[4] = {0};
+    std::vector<double> trace(2 * o.maxit, 0.0);
+    int rc = omx_ir_solve(s, A, lda, o.seed, d, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, out, trace.data());
+    const double stop = now_s();
+    const int steps = (rc == 0) ? (int)out[2] + 1 : o.maxit;
+    for (int i = 0; i < steps; ++i)
+        std::printf("# iterative refinement: step=%d, residual=%e, hpl-harness=%f\n", i, trace[2 * i],
+                    trace[2 * i + 1]);
+    std::printf("\t IR Time %.2f sec\n", stop - start_ir);
+    const double dur = stop - start;
+    const double flops = 2.0 / 3.0 * (double)n * n * n + 3.0 / 2.0 * (double)n * n;
+    const double resid = rc == 0 ? out[0] : -1.0, harness = rc == 0 ? out[1] : -1.0;
+    std::printf("%.3f sec. %.3f GFlop/s resid = %20.15e hpl-harness = %.9f TFlop/s per GPU = %.3f\n", dur,
+                flops / dur * 1e-9, resid, harness, flops / dur * 1e-12);
+    std::printf("===================================\n");
+    std::printf("Code   : omxp\n");
+    std::printf("N      : %ld\nNB     : %ld\nP      : 1\nQ      : 1\n", n, b);
+    std::printf("PFACT  : Right\nDEPTH  : 1\nPREC   : %s\n", fp8 ? "fp8" : "fp16");
+    std::printf("TIME(s): %.3f\nGFLOPS : %.1f\nRESID  : %20.15e\nHARNESS: %.9f\n", dur, flops / dur * 1e-9, resid,
+                harness);
+    omx_solver_destroy(s);
+    (void)hipFree(A);
+    (void)hipFree(d);
+    (void)hipFree(rhs);
+    (void)hipFree(x);
+    return rc == 0 ? 0 : 3;
+}

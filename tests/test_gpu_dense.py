@@ -1,0 +1,116 @@
+"""GPU parity of the FP32 panel kernels (getrf, the three TRSM variants,
+SGEMM, identity, block copy) against the oracle and numpy."""
+import numpy as np
+import pytest
+
+from tests.gpu_util import colmajor_to_numpy, dev, empty, host, ptr, torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+
+def _L():
+    from openmxp_amd import _lib
+    return _lib.lib()
+
+
+def _fdev(M, ld=None):
+    m, n = M.shape
+    ld = m if ld is None else ld
+    buf = np.zeros((n, ld), dtype=np.float32)
+    buf[:, :m] = M.T
+    return dev(buf.ravel()), ld
+
+
+def _dd(n, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(-0.5, 0.5, (n, n)).astype(np.float32)
+    A[np.diag_indices(n)] = np.abs(A).sum(axis=1)
+    return A
+
+
+def test_sgemm_matches_numpy():
+    L = _L()
+    rng = np.random.default_rng(0)
+    m, n, k = 200, 130, 64
+    A = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    B = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+    C = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    Ad, lda = _fdev(A, m + 3)
+    Bd, ldb = _fdev(B, k + 5)
+    Cd, ldc = _fdev(C, m + 1)
+    assert L.omx_sgemm_f32(m, n, k, -1.0, ptr(Ad), lda, ptr(Bd), ldb, 1.0, ptr(Cd), ldc, None) == 0
+    got = colmajor_to_numpy(Cd, m, n, ldc)
+    exact = C.astype(np.float64) - A.astype(np.float64) @ B.astype(np.float64)
+    bound = (k + 2) * 2.0**-24 * (np.abs(C) + np.abs(A).astype(np.float64) @ np.abs(B))
+    assert np.all(np.abs(got - exact) <= bound)
+
+
+@pytest.mark.parametrize("n", [256, 512, 2560])
+def test_getrf_matches_oracle(orc, n):
+    torch = torch_cuda()
+    print(f"getrf n={n}", flush=True)
+    L = _L()
+    A = _dd(n, n)
+    Ad, lda = _fdev(A, n + 64)
+    info = empty(4, torch.int32)
+    assert L.omx_getrf_nopiv_f32(n, ptr(Ad), lda, ptr(info), None) == 0
+    got = colmajor_to_numpy(Ad, n, n, lda).astype(np.float64)
+    assert host(info)[0] == 2**31 - 1
+    want, _ = orc.getrf_nopiv(A)
+    scale = np.abs(want.astype(np.float64)) + 1e-3 * np.abs(want).max() / n
+    assert np.max(np.abs(got - want) / scale) < 1e-3
+    Lf = np.tril(got, -1) + np.eye(n)
+    Uf = np.triu(got)
+    res = np.abs(Lf @ Uf - A.astype(np.float64))
+    assert np.max(res / (np.abs(Lf) @ np.abs(Uf))) < n * 2.0**-23
+
+
+def test_getrf_zero_pivot_info():
+    torch = torch_cuda()
+    L = _L()
+    n = 128
+    A = np.eye(n, dtype=np.float32)
+    A[70, 70] = 0.0
+    Ad, lda = _fdev(A)
+    info = empty(4, torch.int32)
+    assert L.omx_getrf_nopiv_f32(n, ptr(Ad), lda, ptr(info), None) == 0
+    assert host(info)[0] == 71
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_trsm_matches_oracle(orc, variant):
+    L = _L()
+    rng = np.random.default_rng(variant)
+    b, m = 256, 700
+    if variant == 0:
+        T = (np.triu(rng.uniform(-1, 1, (b, b))) + np.eye(b) * 16).astype(np.float32)
+        B = rng.uniform(-1, 1, (m, b)).astype(np.float32)
+        want = orc.trsm_runn(T, B)
+    elif variant == 1:
+        T = (np.tril(rng.uniform(-1, 1, (b, b)), -1) / 8 + np.eye(b)).astype(np.float32)
+        B = rng.uniform(-1, 1, (b, m)).astype(np.float32)
+        want = orc.trsm_llnu(T, B)
+    else:
+        T = (np.tril(rng.uniform(-1, 1, (b, b)), -1) / 8 + np.eye(b)).astype(np.float32)
+        B = rng.uniform(-1, 1, (m, b)).astype(np.float32)
+        want = orc.trsm_rlnu(T, B)
+    Td, ldt = _fdev(T)
+    Bd, ldb = _fdev(B, B.shape[0] + 2)
+    assert L.omx_trsm_f32(variant, B.shape[0], B.shape[1], ptr(Td), ldt, ptr(Bd), ldb, None) == 0
+    got = colmajor_to_numpy(Bd, B.shape[0], B.shape[1], ldb)
+    assert np.max(np.abs(got - want)) <= 1e-5 * max(1.0, np.abs(want).max())
+
+
+def test_identity_and_copy():
+    torch = torch_cuda()
+    L = _L()
+    n = 300
+    I = empty(n * (n + 4), torch.float32)
+    assert L.omx_identity_f32(n, ptr(I), n + 4, None) == 0
+    assert np.array_equal(colmajor_to_numpy(I, n, n, n + 4), np.eye(n, dtype=np.float32))
+    rng = np.random.default_rng(1)
+    M = rng.uniform(-1, 1, (50, 40)).astype(np.float32)
+    Md, ldm = _fdev(M, 60)
+    O = empty(40 * 50, torch.float32)
+    assert L.omx_copy_block_f32(50, 40, ptr(Md), ldm, ptr(O), 50, None) == 0
+    assert np.array_equal(colmajor_to_numpy(O, 50, 40), M)

@@ -1,0 +1,209 @@
+"""GPU parity of the panel packs (FP16, scaled FP8) and of the MFMA Schur GEMM
+against the oracle.  FP16/FP8 products are exact in FP32, so the GEMM is
+checked against an FP64 product of the same rounded operands within the FP32
+accumulation bound k * 2^-24 * |X||Y|^T."""
+import numpy as np
+import pytest
+
+from tests.gpu_util import dev, empty, host, ptr, torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+
+def _L():
+    from openmxp_amd import _lib
+    return _lib.lib()
+
+
+def pk16_index(R, K):
+    Rp = -(-R // 256) * 256
+    r = np.arange(Rp)[:, None]
+    k = np.arange(K)[None, :]
+    return (((r >> 8) * (K >> 6) + (k >> 6)) * 16384 + ((((r & 255) >> 5) << 2) + ((k & 63) >> 4)) * 512
+            + (((r & 31) + 32 * ((k & 15) >> 3)) << 3) + (k & 7))
+
+
+def pk8_index(R, K):
+    Rp = -(-R // 256) * 256
+    r = np.arange(Rp)[:, None]
+    k = np.arange(K)[None, :]
+    return (((r >> 8) * (K >> 7) + (k >> 7)) * 32768 + ((((r & 255) >> 5) << 1) + ((k & 127) >> 6)) * 2048
+            + ((k & 31) >> 4) * 1024 + (((r & 31) + 32 * ((k & 63) >> 5)) << 4) + (k & 15))
+
+
+def pack16(Xh):
+    """numpy FP16 (R x K) -> packed uint16 array (reference packing)."""
+    R, K = Xh.shape
+    idx = pk16_index(R, K)
+    out = np.zeros(idx.size, dtype=np.uint16)
+    full = np.zeros((idx.shape[0], K), dtype=np.uint16)
+    full[:R] = Xh.view(np.uint16)
+    out[idx.ravel()] = full.ravel()
+    return out
+
+
+def test_pack_index_is_a_permutation():
+    for R, K in [(256, 64), (300, 128), (512, 2560)]:
+        idx = pk16_index(R, K).ravel()
+        assert np.array_equal(np.sort(idx), np.arange(idx.size))
+    for R, K in [(256, 128), (300, 512)]:
+        idx = pk8_index(R, K).ravel()
+        assert np.array_equal(np.sort(idx), np.arange(idx.size))
+
+
+@pytest.mark.parametrize("side", ["left", "right"])
+@pytest.mark.parametrize("R,K", [(300, 128), (512, 256)])
+def test_pack_f16_bitexact(R, K, side):
+    torch = torch_cuda()
+    L = _L()
+    rng = np.random.default_rng(R + K)
+    S = (rng.standard_normal((R, K)) * 10.0 ** rng.uniform(-6, 2, (R, K))).astype(np.float32)
+    if side == "left":   # column-major R x K block with ld = R + 7
+        ld = R + 7
+        src = np.zeros((K, ld), dtype=np.float32)
+        src[:, :R] = S.T
+        rs, ls = 1, ld
+    else:                # S(r, k) = M(k, r) of a column-major K x R block, ld = K + 4
+        ld = K + 4
+        src = np.zeros((R, ld), dtype=np.float32)
+        src[:, :K] = S
+        rs, ls = ld, 1
+    sd = dev(src.ravel())
+    nbytes = L.omx_panel_bytes(R, K, 0)
+    P = empty(nbytes // 2, torch.int16)
+    cm = empty((-(-R // 256)) * (K // 64), torch.float32)
+    assert L.omx_pack_f16(R, K, ptr(sd), rs, ls, 0, ptr(P), ptr(cm), None) == 0
+    got = host(P).view(np.uint16)
+    want = pack16(S.astype(np.float16))
+    assert np.array_equal(got, want)
+    # chunk max-abs statistics (of the FP32 source values)
+    cmx = host(cm).reshape(-(-R // 256), K // 64)
+    Sp = np.zeros((cmx.shape[0] * 256, K), np.float32)
+    Sp[:R] = np.abs(S)
+    assert np.array_equal(cmx, Sp.reshape(cmx.shape[0], 256, K // 64, 64).max(axis=(1, 3)))
+
+
+@pytest.mark.parametrize("mask", [1, 2, 3])
+def test_pack_f16_masks(mask):
+    torch = torch_cuda()
+    L = _L()
+    K = 256
+    rng = np.random.default_rng(mask)
+    W = rng.uniform(-1, 1, (K, K)).astype(np.float32)
+    sd = dev(np.asfortranarray(W).ravel(order="F"))
+    P = empty(L.omx_panel_bytes(K, K, 0) // 2, torch.int16)
+    assert L.omx_pack_f16(K, K, ptr(sd), 1, K, mask, ptr(P), None, None) == 0
+    r = np.arange(K)[:, None]
+    k = np.arange(K)[None, :]
+    if mask == 1:
+        M = np.where(k <= r, W, 0)
+    elif mask == 2:
+        M = np.where(k > r, W, np.where(k == r, 1.0, 0.0))
+    else:
+        M = np.where(r > k, W, np.where(r == k, 1.0, 0.0))
+    assert np.array_equal(host(P).view(np.uint16), pack16(M.astype(np.float16)))
+
+
+@pytest.mark.parametrize("side", ["left", "right"])
+def test_pack_f8_bitexact(orc, side):
+    torch = torch_cuda()
+    L = _L()
+    R, K = 300, 512
+    rng = np.random.default_rng(11)
+    S = (rng.standard_normal((R, K)) * 1e-5).astype(np.float32)
+    S[:, 256:] *= 3e4  # a second scale tile with a different exponent
+    if side == "left":
+        src, rs, ls = np.asfortranarray(S).ravel(order="F"), 1, R
+    else:
+        src, rs, ls = np.ascontiguousarray(S).ravel(), K, 1
+    sd = dev(src)
+    P = empty(L.omx_panel_bytes(R, K, 1), torch.uint8)
+    ne = L.omx_panel_exps_count(R, K)
+    ex = empty(ne, torch.int32)
+    tmx = empty(ne, torch.float32)
+    assert L.omx_pack_f8(R, K, ptr(sd), rs, ls, ptr(P), ptr(ex), ptr(tmx), None) == 0
+    e = host(ex).reshape(2, 2)
+    for i in range(2):
+        for j in range(2):
+            blk = S[i * 256:(i + 1) * 256, j * 256:(j + 1) * 256]
+            assert e[i, j] == orc.pow2_scale_exp(float(np.abs(blk).max()))
+    scaled = np.zeros((512, K), np.float32)
+    for i in range(2):
+        for j in range(2):
+            blk = S[i * 256:(i + 1) * 256, j * 256:(j + 1) * 256]
+            scaled[i * 256:i * 256 + blk.shape[0], j * 256:(j + 1) * 256] = np.ldexp(blk, e[i, j])
+    q = orc.f32_to_e4m3_bits(scaled)
+    idx = pk8_index(R, K)
+    want = np.zeros(idx.size, np.uint8)
+    want[idx.ravel()] = q.ravel()
+    assert np.array_equal(host(P), want)
+
+
+def _gemm_case(m, n, k, alpha, beta, ldc_pad=3, x_row0=0, y_row0=0, seed=0):
+    torch = torch_cuda()
+    L = _L()
+    rng = np.random.default_rng(seed)
+    Xh = rng.uniform(-1, 1, (x_row0 + m, k)).astype(np.float16)
+    Yh = rng.uniform(-1, 1, (y_row0 + n, k)).astype(np.float16)
+    ldc = m + ldc_pad
+    C0 = rng.uniform(-1, 1, (n, ldc)).astype(np.float32)  # column-major m x n, ld = ldc
+    Xd, Yd, Cd = dev(pack16(Xh).view(np.int16)), dev(pack16(Yh).view(np.int16)), dev(C0.ravel())
+    rc = L.omx_gemm_f16_packed(m, n, k, alpha, ptr(Xd), x_row0, ptr(Yd), y_row0, beta, ptr(Cd), ldc, None)
+    assert rc == 0
+    got = host(Cd).reshape(n, ldc)
+    X64 = Xh[x_row0:].astype(np.float64)
+    Y64 = Yh[y_row0:].astype(np.float64)
+    Cm = C0[:, :m].T.astype(np.float64)
+    exact = (beta * Cm if beta != 0 else 0) + alpha * (X64 @ Y64.T)
+    bound = (k + 2) * 2.0**-24 * np.abs(alpha) * (np.abs(X64) @ np.abs(Y64).T) + 2.0**-23 * np.abs(exact)
+    assert np.all(np.abs(got[:, :m].T - exact) <= bound)
+    assert np.array_equal(got[:, m:], C0[:, m:])  # padding rows of C untouched
+
+
+@pytest.mark.parametrize("m,n,k", [(256, 256, 64), (512, 768, 256), (300, 200, 128), (2560, 2560, 2560)])
+def test_gemm_f16_schur(m, n, k):
+    print(f"gemm_f16 {m}x{n}x{k}", flush=True)
+    _gemm_case(m, n, k, -1.0, 1.0, seed=m + n)
+
+
+def test_gemm_f16_beta_zero_and_offsets():
+    _gemm_case(512, 256, 256, 1.0, 0.0, x_row0=256, y_row0=512, seed=5)
+    _gemm_case(1024, 1280, 512, -1.0, 1.0, x_row0=768, y_row0=256, ldc_pad=100, seed=6)
+
+
+def test_gemm_f8_schur(orc):
+    torch = torch_cuda()
+    L = _L()
+    m, n, k = 512, 768, 512
+    rng = np.random.default_rng(3)
+    Xs = (rng.standard_normal((m, k)) * 1e-5).astype(np.float32)
+    Ys = rng.uniform(-0.5, 0.5, (n, k)).astype(np.float32)
+    packs = []
+    for S in (Xs, Ys):
+        R = S.shape[0]
+        sd = dev(np.asfortranarray(S).ravel(order="F"))
+        P = empty(L.omx_panel_bytes(R, k, 1), torch.uint8)
+        ne = L.omx_panel_exps_count(R, k)
+        ex = empty(ne, torch.int32)
+        tmx = empty(ne, torch.float32)
+        assert L.omx_pack_f8(R, k, ptr(sd), 1, R, ptr(P), ptr(ex), ptr(tmx), None) == 0
+        packs.append((P, ex, sd))
+    C0 = rng.uniform(-1, 1, (n, m)).astype(np.float32)
+    Cd = dev(C0.ravel())
+    (XP, xe, _), (YP, ye, _) = packs
+    assert L.omx_gemm_f8_packed(m, n, k, -1.0, ptr(XP), ptr(xe), 0, ptr(YP), ptr(ye), 0, 1.0, ptr(Cd), m, None) == 0
+    got = host(Cd).reshape(n, m).T.astype(np.float64)
+    # decode the packed operands through the oracle and form the exact product
+    dec = []
+    for (P, ex, _), S in zip(packs, (Xs, Ys)):
+        R = S.shape[0]
+        q = host(P)[pk8_index(R, k)][:R]
+        e = host(ex).reshape(-(-R // 256), k // 256)
+        scale = np.ldexp(1.0, -np.repeat(np.repeat(e, 256, 0), 256, 1)[:R])
+        dec.append(orc.e4m3_bits_to_f32(q).astype(np.float64) * scale)
+    exact = C0.T.astype(np.float64) - dec[0] @ dec[1].T
+    bound = (k + 2) * 2.0**-24 * (np.abs(dec[0]) @ np.abs(dec[1]).T) + 2.0**-23 * np.abs(exact)
+    assert np.all(np.abs(got - exact) <= bound)
+    # and the FP8 result approximates the FP32 product to e4m3 accuracy
+    full = C0.T.astype(np.float64) - Xs.astype(np.float64) @ Ys.astype(np.float64).T
+    assert np.max(np.abs(got - full)) < 0.1 * np.max(np.abs(Xs.astype(np.float64) @ Ys.T.astype(np.float64)))

@@ -66,7 +66,9 @@ def test_norm_and_vector_ops_bitexact():
     assert L.omx_axpy_f64(n, 0.75, ptr(ad), ptr(bd), None) == 0
     got = host(bd)
     exact = b + 0.75 * a
-    assert np.all(np.abs(got - exact) <= 2.0**-52 * np.abs(exact))
+    # the device contracts y + alpha*x into one fma: one rounding instead of
+    # numpy's two, so they agree to within an ulp of the summands
+    assert np.all(np.abs(got - exact) <= 2.0**-52 * (np.abs(b) + 0.75 * np.abs(a)))
 
 
 @pytest.mark.parametrize("n,b", [(1024, 256), (2048, 512)])

@@ -1,0 +1,181 @@
+"""End-to-end GPU parity: the look-ahead LU against the oracle's LU in the
+reference step order, and the GPU refinement (classical and GMRES-IR)
+against the oracle and an FP64 LAPACK solve."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from tests.gpu_util import colmajor_to_numpy, empty, host, ptr, torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+
+def _L():
+    from openmxp_amd import _lib
+    return _lib.lib()
+
+
+class Solver:
+    def __init__(self, n, b, alt=1, fp8=False):
+        self.L = _L()
+        self.h = C.c_void_p()
+        rc = self.L.omx_solver_create(n, b, alt, 1 if fp8 else 0, C.byref(self.h))
+        assert rc == 0, rc
+        self.n, self.b = n, b
+
+    def lu(self, A, lda, sched=None):
+        tim = np.zeros(5)
+        sp = None
+        if sched is not None:
+            s = np.ascontiguousarray(sched, dtype=np.int32)
+            sp = s.ctypes.data
+        info = self.L.omx_lu_factor(self.h, ptr(A), lda, sp, tim.ctypes.data)
+        return info, tim
+
+    def ir(self, LU, lda, seed, d, b, x, method=0, maxit=50, restart=50, tol=1e-10):
+        out = np.zeros(4)
+        trace = np.zeros(2 * maxit)
+        rc = self.L.omx_ir_solve(self.h, ptr(LU), lda, seed, ptr(d), ptr(b), ptr(x), method, maxit, restart,
+                                 tol, out.ctypes.data, trace.ctypes.data)
+        return rc, out, trace
+
+    def close(self):
+        self.L.omx_solver_destroy(self.h)
+
+
+def _generate(n, seed, lda):
+    torch = torch_cuda()
+    L = _L()
+    A = empty(lda * n, torch.float32)
+    d = empty(n, torch.float64)
+    b = empty(n, torch.float64)
+    assert L.omx_matgen_f32(n, seed, ptr(A), lda, ptr(d), None) == 0
+    assert L.omx_rhs_f64(n, seed, ptr(b), None) == 0
+    return A, d, b
+
+
+@pytest.mark.parametrize("alt", [0, 1, 2])
+def test_lu_matches_oracle(orc, alt):
+    torch = torch_cuda()
+    n, b, seed = 1536, 256, 42
+    print(f"lu n={n} b={b} alt={alt}", flush=True)
+    lda = n + 64
+    A, d, rhs = _generate(n, seed, lda)
+    S = Solver(n, b, alt)
+    info, tim = S.lu(A, lda)
+    S.close()
+    assert info == 0
+    got = colmajor_to_numpy(A, n, n, lda).astype(np.float64)
+    Ao, _ = orc.matgen_full(n, seed)
+    want, _ = orc.lu_factor(Ao, b, alt=alt)
+    want = want.astype(np.float64)
+    low = np.tril(np.ones((n, n)), -1).astype(bool)
+    dl = np.abs(got - want)[low]
+    du = np.abs(got - want)[~low]
+    lmax = np.abs(want[low]).max()
+    print(f"  max |dL|/max|L| = {dl.max() / lmax:.3e}, max |dU|/(|U|+.05) = "
+          f"{np.max(du / (np.abs(want[~low]) + 0.05)):.3e}", flush=True)
+    # FP16 Schur inputs: factors agree to a few FP16 ulps of the panel entries
+    assert dl.max() <= 2.0**-8 * lmax
+    assert np.max(du / (np.abs(want[~low]) + 0.05)) <= 2.0**-8
+    # backward error of the GPU factors within the FP16 bound
+    Lf = np.tril(got, -1) + np.eye(n)
+    Uf = np.triu(got)
+    R = Lf @ Uf - Ao.astype(np.float64)
+    # (FP16 panel inputs and, for -alt 1/2, an FP16 inverse: a few units of 2^-11)
+    assert np.max(np.abs(R) / (2.0**-9 * (np.abs(Lf) @ np.abs(Uf)))) < 1.0
+    assert tim[2] == n // b - 2 and tim[1] > 0
+
+
+def known_factors(n, seed):
+    """Well-conditioned L0 (unit lower, entries in [0, 1/n)) and diagonally
+    dominant U0 with coherent signs: each Schur block update is large enough
+    that dropping or misplacing one changes U by ~0.06, against rounding
+    differences below 1e-3."""
+    rng = np.random.default_rng(seed)
+    L0 = np.tril(rng.uniform(0, 1, (n, n)) / n, -1) + np.eye(n)
+    U0 = np.triu(rng.uniform(0, 1, (n, n)), 1)
+    U0 += np.diag(1.0 + U0.sum(axis=1))
+    return L0, U0
+
+
+@pytest.mark.parametrize("alt", [0, 1, 2])
+def test_lu_recovers_known_factors(alt):
+    torch = torch_cuda()
+    from tests.gpu_util import dev
+    n, b = 1280, 256
+    L0, U0 = known_factors(n, alt + 10)
+    A64 = L0 @ U0
+    Ad = dev(np.asfortranarray(A64.astype(np.float32)).ravel(order="F"))
+    S = Solver(n, b, alt)
+    info, _ = S.lu(Ad, n)
+    S.close()
+    assert info == 0
+    got = colmajor_to_numpy(Ad, n, n).astype(np.float64)
+    Lg = np.tril(got, -1) + np.eye(n)
+    Ug = np.triu(got)
+    print(f"  alt={alt}: max|L-L0| = {np.abs(Lg - L0).max():.2e}, max|U-U0| = {np.abs(Ug - U0).max():.2e}",
+          flush=True)
+    assert np.abs(Lg - L0).max() < 2e-5
+    assert np.abs(Ug - U0).max() < 5e-3
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_refinement_matches_oracle_and_lapack(orc, method):
+    torch = torch_cuda()
+    n, b, seed = 1024, 256, 7
+    lda = n
+    A, d, rhs = _generate(n, seed, lda)
+    S = Solver(n, b, 1)
+    info, _ = S.lu(A, lda)
+    assert info == 0
+    x = empty(n, torch.float64)
+    rc, out, trace = S.ir(A, lda, seed, d, rhs, x, method=method)
+    S.close()
+    assert rc == 0, (out, trace)
+    assert out[1] < 16.0
+    LU = colmajor_to_numpy(A, n, n, lda)
+    dd = host(d)
+    bb = host(rhs)
+    res = (orc.ir_classic if method == 0 else orc.gmres_ir)(LU, b, seed, dd, bb)
+    assert res["iters"] >= 0
+    assert abs(int(out[2]) - res["iters"]) <= 1
+    # FP64 LAPACK solution of the same generated system
+    A64 = np.array([[orc.lcg_value(orc.lcg_state(seed, n, i, j)) for j in range(n)] for i in range(n)])
+    A64[np.diag_indices(n)] = dd
+    xs = sla.solve(A64, bb)
+    xg = host(x)
+    kappa = np.linalg.cond(A64, np.inf)
+    fwd = np.max(np.abs(xg - xs)) / np.max(np.abs(xs))
+    print(f"  method={method} steps={int(out[2])} harness={out[1]:.4f} fwd={fwd:.3e}", flush=True)
+    assert fwd <= kappa * n * 2.0**-53 * (2 * max(out[1], 1.0) + 1)
+
+
+def test_configs0_solve_converges():
+    """N=10240, B=256 (the reference's CPU-runnable case), classical IR."""
+    torch = torch_cuda()
+    n, b, seed = 10240, 256, 42
+    print("solve n=10240 b=256", flush=True)
+    A, d, rhs = _generate(n, seed, n)
+    S = Solver(n, b, 1)
+    info, tim = S.lu(A, n)
+    x = empty(n, torch.float64)
+    rc, out, trace = S.ir(A, n, seed, d, rhs, x)
+    S.close()
+    print(f"  lu {tim[0]:.1f} ms, steps {int(out[2])}, harness {out[1]:.4f}", flush=True)
+    assert info == 0 and rc == 0 and out[1] < 16.0 and out[2] <= 5
+
+
+def test_fp8_schedule_converges():
+    torch = torch_cuda()
+    n, b, seed = 2048, 256, 42
+    A, d, rhs = _generate(n, seed, n)
+    S = Solver(n, b, 1, fp8=True)
+    info, _ = S.lu(A, n, sched=np.ones(n // b, dtype=np.int32))
+    x = empty(n, torch.float64)
+    rc, out, trace = S.ir(A, n, seed, d, rhs, x, method=1)
+    S.close()
+    print(f"  fp8: steps {int(out[2])} harness {out[1]:.4f} inner {int(out[3])}", flush=True)
+    assert info == 0 and rc == 0 and out[1] < 16.0

