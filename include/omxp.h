@@ -266,14 +266,19 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched,
  * iterative_refinement.hpp:141-171, x0 = rhs./diag, stop at harness < 16);
  * method 1: GMRES-IR, the same outer loop with each correction solved by
  * GMRES(restart) on (LU)^-1 A d = (LU)^-1 r to relative tolerance
- * inner_tol.  Blocks until done.  out (host, 4 doubles): final residual,
+ * inner_tol.  The loop stops when the harness falls below stop_harness
+ * (16 in the reference, iterative_refinement.hpp:163); stop_harness <= 0
+ * runs exactly maxit residual evaluations (refinement to the FP64 floor).
+ * x_given = 1 starts from x instead of x0 = rhs./diag.
+ * Blocks until done.  out (host, 4 doubles): final residual,
  * harness, corrections applied (-1 if not converged), total inner
  * iterations.  trace (host, 2*maxit, optional): residual and harness per
  * step.  Returns 0, or OMX_ERR_NOCONV. */
 int omx_ir_solve(omx_solver *s, const float *LU, int64_t lda, uint64_t seed,
                  const double *diag, const double *rhs, double *x,
                  int method, int maxit, int restart, double inner_tol,
-                 double *out, double *trace);
+                 double stop_harness, int x_given, double *out,
+                 double *trace);
 
 /* hpl-harness ||r||/(||A|| ||x|| + ||b||)/(n * DBL_EPSILON/2) on the host
  * (iterative_refinement.hpp:154-157). */

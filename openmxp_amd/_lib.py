@@ -50,7 +50,7 @@ SIGNATURES = {
     "omx_solver_create": (i32, [i64, i64, i32, i32, C.POINTER(vp)]),
     "omx_solver_destroy": (i32, [vp]),
     "omx_lu_factor": (i32, [vp, vp, i64, vp, vp]),
-    "omx_ir_solve": (i32, [vp, vp, i64, u64, vp, vp, vp, i32, i32, i32, dbl, vp, vp]),
+    "omx_ir_solve": (i32, [vp, vp, i64, u64, vp, vp, vp, i32, i32, i32, dbl, dbl, i32, vp, vp]),
     "omx_harness": (dbl, [dbl, dbl, dbl, dbl, i64]),
 }
 

@@ -153,7 +153,8 @@ int main(int argc, char **argv) {
     const double start_ir = now_s();
     double out[4] = {0};
     std::vector<double> trace(2 * o.maxit, 0.0);
-    int rc = omx_ir_solve(s, A, lda, o.seed, d, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, out, trace.data());
+    int rc = omx_ir_solve(s, A, lda, o.seed, d, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, 16.0, 0, out,
+                          trace.data());
     const double stop = now_s();
     const int steps = (rc == 0) ? (int)out[2] + 1 : o.maxit;
     for (int i = 0; i < steps; ++i)

@@ -38,7 +38,7 @@ class Solver:
         out = np.zeros(4)
         trace = np.zeros(2 * maxit)
         rc = self.L.omx_ir_solve(self.h, ptr(LU), lda, seed, ptr(d), ptr(b), ptr(x), method, maxit, restart,
-                                 tol, out.ctypes.data, trace.ctypes.data)
+                                 tol, 16.0, 0, out.ctypes.data, trace.ctypes.data)
         return rc, out, trace
 
     def close(self):
