@@ -8,17 +8,22 @@
 //
 // Workgroup tile 256 x 256, 4 waves in a 2 x 2 arrangement, each wave owns
 // 128 x 128 of C as 4 x 4 accumulators of a 32 x 32 MFMA (256 FP32
-// registers).  K advances in 32 KB stages per operand (64 FP16 or 128 FP8
-// columns); two stages are double-buffered in LDS (128 KB).  Global -> LDS
-// copies are direct (global_load_lds_dwordx4): the packed panel chunk is the
-// LDS image, so no register staging is needed and every MFMA operand is one
-// conflict-free ds_read_b128 per lane (two for FP8).
+// registers).  K advances in LDS stages of 16 KB per operand (32 FP16 or 64
+// FP8 columns of the 256-row strips); OMX_GEMM_NBUF stages form a ring in LDS
+// (128 KB for 4) with all but one in flight.  Global -> LDS copies are direct
+// (global_load_lds_dwordx4): the packed panel chunk is the LDS image, so no
+// register staging is needed and every MFMA operand is one conflict-free
+// ds_read_b128 per lane (two for FP8).
 //
 // The MFMA's A operand is the Y fragment and its B operand the X fragment,
 // so in the 32 x 32 result lane l holds row i = l%32 of C and 16 columns:
 // each store of a register writes 32 consecutive rows of one column.
 #include "omx_common.hpp"
 #include "omx_layout.hpp"
+
+#ifndef OMX_GEMM_NBUF
+#define OMX_GEMM_NBUF 4
+#endif
 
 namespace {
 
@@ -33,14 +38,16 @@ constexpr int kChunk = 32768;  // bytes per operand per stage
 constexpr int kGroupM = 8;     // row strips per group of the tile order
 
 // Workgroup -> (row tile, column tile).  Workgroups are dispatched
-// round-robin over the 8 XCDs; consecutive logical ids are first gathered
-// on one XCD (so one XCD's L2 sees a compact block of tiles), then ordered
-// in groups of kGroupM row strips, column by column, so the workgroups in
-// flight share X and Y strips.
+// round-robin over the 8 XCDs (workgroup id % 8); the ids are first remapped
+// so that each XCD owns a contiguous range of logical tiles (its L2 then
+// sees a compact block of the tile space), and logical tiles are ordered in
+// groups of kGroupM row strips, column by column, so the workgroups in
+// flight on one XCD share X and Y strips.
 __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int &tm, int &tn) {
     const int total = tiles_m * tiles_n;
-    int logical = bid;
-    if ((total & 7) == 0) logical = (bid & 7) * (total >> 3) + (bid >> 3);
+    const int xcd = bid & 7, idx = bid >> 3;
+    const int q = total >> 3, r = total & 7;
+    const int logical = (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
     const int gsize = kGroupM * tiles_n;
     const int g = logical / gsize;
     const int first = g * kGroupM;
@@ -50,24 +57,88 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int &
     tn = in / gm;
 }
 
-__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
-
-// Copy one 32 KB chunk global -> LDS; wave w moves bytes [8w KB, 8(w+1) KB).
-__device__ __forceinline__ void load_chunk(const char *__restrict__ g, char *l, int wave, int lane) {
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        const int seg = wave * 8 + s;
-        __builtin_amdgcn_global_load_lds(reinterpret_cast<const void *>(g + seg * 1024 + lane * 16),
-                                         (lds_void *)(l + seg * 1024), 16, 0, 0);
+// s_waitcnt vmcnt(8 * stages): wait until at most `stages` later stages
+// (8 loads per wave each) are still in flight.  gfx9 encoding of the
+// immediate: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt[5:4] << 14;
+// expcnt and lgkmcnt are left at their maxima (no wait).
+__device__ __forceinline__ void wait_stages(int stages) {
+    switch (stages) {
+        case 0: __builtin_amdgcn_s_waitcnt(0x0F70); break;   // vmcnt(0)
+        case 1: __builtin_amdgcn_s_waitcnt(0x0F78); break;   // vmcnt(8)
+        case 2: __builtin_amdgcn_s_waitcnt(0x4F70); break;   // vmcnt(16)
+        default: __builtin_amdgcn_s_waitcnt(0x4F78); break;  // vmcnt(24)
     }
+}
+
+__device__ __forceinline__ void gl_lds16(const char *g, char *l) {
+    __builtin_amdgcn_global_load_lds(reinterpret_cast<const void *>(g), (lds_void *)l, 16, 0, 0);
+}
+
+// C update of one wave's 128 x 128 block.  acc[mi][ni][r] is
+//   C(i0 + 32 mi + lane%32, j0 + 32 ni + 8 (r/4) + 4 (lane/32) + r%4),
+// so each lane owns 16 columns of one row per 32 x 32 block and every load or
+// store instruction covers 32 consecutive rows of two columns.  Interior
+// tiles go through buffer instructions, one resource per 32-column block
+// (wave-uniform 64-bit base in SGPRs, 16 per-lane 32-bit column offsets, the
+// row block in the immediate), pipelined over ni: the 64 old values of
+// column block ni + 1 are in flight while column block ni is combined and
+// stored.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t colblock_rsrc(float *p, int64_t ldc) {
+    return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)(ldc * 32 * 4), 0x00020000);
 }
 
 template <bool kBetaZero>
 __device__ __forceinline__ void epilogue(floatx16 (&acc)[4][4], int64_t M, int64_t N, float alpha,
                                          float beta, float *__restrict__ C, int64_t ldc, int tm,
                                          int tn, int wm, int wn, int lane) {
-    const int64_t ib = (int64_t)tm * OMX_STRIP + wm * 128 + (lane & 31);
-    const int64_t jb = (int64_t)tn * OMX_STRIP + wn * 128 + 4 * (lane >> 5);
+    const int64_t i0 = (int64_t)tm * OMX_STRIP + wm * 128;
+    const int64_t j0 = (int64_t)tn * OMX_STRIP + wn * 128;
+    if ((int64_t)(tm + 1) * OMX_STRIP <= M && (int64_t)(tn + 1) * OMX_STRIP <= N &&
+        ldc * 32 * 4 < ((int64_t)1 << 31)) {
+        int voff[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            voff[r] = (int)(((8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * ldc + (lane & 31)) * 4);
+        if (kBetaZero) {
+#pragma unroll
+            for (int ni = 0; ni < 4; ++ni) {
+                const __amdgpu_buffer_rsrc_t rs = colblock_rsrc(C + (j0 + 32 * ni) * ldc + i0, ldc);
+#pragma unroll
+                for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, alpha * acc[mi][ni][r]),
+                                                              rs, voff[r] + mi * 128, 0, 0);
+            }
+            return;
+        }
+        float old[2][4][16];
+        auto fetch = [&](float(&o)[4][16], int ni) {
+            const __amdgpu_buffer_rsrc_t rs = colblock_rsrc(C + (j0 + 32 * ni) * ldc + i0, ldc);
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    o[mi][r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff[r] + mi * 128, 0, 0));
+        };
+        fetch(old[0], 0);
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) {
+            if (ni + 1 < 4) fetch(old[(ni + 1) & 1], ni + 1);
+            const __amdgpu_buffer_rsrc_t rs = colblock_rsrc(C + (j0 + 32 * ni) * ldc + i0, ldc);
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    __builtin_amdgcn_raw_buffer_store_b32(
+                        __builtin_bit_cast(uint32_t, beta * old[ni & 1][mi][r] + alpha * acc[mi][ni][r]), rs,
+                        voff[r] + mi * 128, 0, 0);
+        }
+        return;
+    }
+    // edge tiles (and ldc too large for 32-bit column offsets): checked
+    const int64_t ib = i0 + (lane & 31);
+    const int64_t jb = j0 + 4 * (lane >> 5);
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi) {
         const int64_t i = ib + mi * 32;
@@ -95,18 +166,50 @@ __device__ __forceinline__ void epilogue(floatx16 (&acc)[4][4], int64_t M, int64
     }
 }
 
-template <bool kBetaZero>
+// ---------------------------------------------------------------------------
+// FP16: LDS stages of 32 columns (16 KB per operand), kNbuf of them in a
+// ring, kNbuf-1 in flight.  Stage s of a 256-row strip is half (s & 1) of
+// packed chunk s/2: fragment blocks rr*4 + 2*(s&1) + {0,1}, copied to the
+// LDS stage as 16 blocks of 1 KB in [rr][k-step] order.  Per stage each wave
+// runs two k-steps of 16 MFMAs; between them it waits for the next stage,
+// passes the single barrier of the stage, refills the ring slot freed by the
+// previous stage and reads the next stage's first fragments, which then
+// arrive while the second 16 MFMAs run.
+// ---------------------------------------------------------------------------
+template <bool kBetaZero, int kNbuf>
 __global__ void __launch_bounds__(kThreads, 1)
 k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *__restrict__ X,
            const char *__restrict__ Y, float *__restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
-    __shared__ __attribute__((aligned(1024))) char smem[2][2][kChunk];
+    __shared__ __attribute__((aligned(1024))) char smem[kNbuf][2][16384];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     int tm, tn;
     tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    const int64_t nks = K / OMX_KSTAGE16;
-    const char *xs = X + (int64_t)tm * nks * kChunk;
-    const char *ys = Y + (int64_t)tn * nks * kChunk;
+    const int nks = (int)(K / 32);
+    const char *xs = X + (int64_t)tm * (K / OMX_KSTAGE16) * kChunk;
+    const char *ys = Y + (int64_t)tn * (K / OMX_KSTAGE16) * kChunk;
+
+    auto issue = [&](int st) {
+        const int64_t base = (int64_t)(st >> 1) * kChunk + (st & 1) * 2048;
+        char *lx = smem[st % kNbuf][0];
+        char *ly = smem[st % kNbuf][1];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int seg = wave * 4 + t;
+            const int64_t off = base + (seg >> 1) * 4096 + (seg & 1) * 1024 + lane * 16;
+            gl_lds16(xs + off, lx + seg * 1024);
+            gl_lds16(ys + off, ly + seg * 1024);
+        }
+    };
+    auto frags = [&](half8(&xd)[4], half8(&yd)[4], int st, int kk) {
+        const char *lx = smem[st % kNbuf][0];
+        const char *ly = smem[st % kNbuf][1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            xd[q] = *reinterpret_cast<const half8 *>(lx + ((wm * 4 + q) * 2 + kk) * 1024 + lane * 16);
+            yd[q] = *reinterpret_cast<const half8 *>(ly + ((wn * 4 + q) * 2 + kk) * 1024 + lane * 16);
+        }
+    };
 
     floatx16 acc[4][4];
 #pragma unroll
@@ -114,54 +217,101 @@ k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char 
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = floatx16{};
 
-    load_chunk(xs, smem[0][0], wave, lane);
-    load_chunk(ys, smem[0][1], wave, lane);
-    wait_vmem();
+    for (int st = 0; st < kNbuf - 1 && st < nks; ++st) issue(st);
+    wait_stages(min(kNbuf - 2, nks - 1));
     __syncthreads();
-    for (int64_t ks = 0; ks < nks; ++ks) {
-        const int buf = (int)(ks & 1);
-        if (ks + 1 < nks) {
-            load_chunk(xs + (ks + 1) * kChunk, smem[buf ^ 1][0], wave, lane);
-            load_chunk(ys + (ks + 1) * kChunk, smem[buf ^ 1][1], wave, lane);
+    half8 xa[4], ya[4], xb[4], yb[4];
+    frags(xa, ya, 0, 0);
+    // The LDS-DMA loads make the compiler wait for all LDS reads before an
+    // MFMA that uses any of them, so the next fragments are read in between
+    // the current MFMAs (2 MFMAs, 1 read): they have landed when needed.
+    for (int st = 0; st < nks; ++st) {
+        frags(xb, yb, st, 1);
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < 4; ++ni)
+                acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ya[ni], xa[mi], acc[mi][ni], 0, 0, 0);
+#pragma unroll
+        for (int g = 0; g < 8; ++g) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // 2 MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 DS read
         }
-        const _Float16 *xl = reinterpret_cast<const _Float16 *>(smem[buf][0]);
-        const _Float16 *yl = reinterpret_cast<const _Float16 *>(smem[buf][1]);
+        __builtin_amdgcn_sched_barrier(0);
+        const bool more = st + 1 < nks;
+        if (more) {
+            wait_stages(min(kNbuf - 3, nks - st - 2));
+            __syncthreads();
+            if (st + kNbuf - 1 < nks) issue(st + kNbuf - 1);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (more) frags(xa, ya, st + 1, 0);
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-            half8 xf[4], yf[4];
+        for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                xf[q] = *reinterpret_cast<const half8 *>(xl + (((wm * 4 + q) * 4 + kk) * 512 + lane * 8));
-                yf[q] = *reinterpret_cast<const half8 *>(yl + (((wn * 4 + q) * 4 + kk) * 512 + lane * 8));
+            for (int ni = 0; ni < 4; ++ni)
+                acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(yb[ni], xb[mi], acc[mi][ni], 0, 0, 0);
+        if (more) {
+#pragma unroll
+            for (int g = 0; g < 8; ++g) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
             }
-#pragma unroll
-            for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-                for (int ni = 0; ni < 4; ++ni)
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(yf[ni], xf[mi], acc[mi][ni], 0, 0, 0);
         }
-        wait_vmem();
-        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
     }
     epilogue<kBetaZero>(acc, M, N, alpha, beta, C, ldc, tm, tn, wm, wn, lane);
 }
 
-template <bool kBetaZero>
+// ---------------------------------------------------------------------------
+// FP8: LDS stages of 64 columns (16 KB per operand): stage s is fragment
+// blocks rr*2 + (s&1) (2 KB each, two 1 KB halves) of packed chunk s/2.  One
+// k-step of 16 scaled MFMAs per stage, split 8 + 8 around the barrier.
+// ---------------------------------------------------------------------------
+template <bool kBetaZero, int kNbuf>
 __global__ void __launch_bounds__(kThreads, 1)
 k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *__restrict__ X,
           const int32_t *__restrict__ xexp, const char *__restrict__ Y, const int32_t *__restrict__ yexp,
           float *__restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
-    __shared__ __attribute__((aligned(1024))) char smem[2][2][kChunk];
+    __shared__ __attribute__((aligned(1024))) char smem[kNbuf][2][16384];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     int tm, tn;
     tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    const int64_t nks = K / OMX_KSTAGE8;
+    const int nks = (int)(K / 64);
     const int64_t KT = K / OMX_SCALE_TILE;
-    const char *xs = X + (int64_t)tm * nks * kChunk;
-    const char *ys = Y + (int64_t)tn * nks * kChunk;
+    const char *xs = X + (int64_t)tm * (K / OMX_KSTAGE8) * kChunk;
+    const char *ys = Y + (int64_t)tn * (K / OMX_KSTAGE8) * kChunk;
     const int32_t *xe = xexp + (int64_t)tm * KT;
     const int32_t *ye = yexp + (int64_t)tn * KT;
+
+    auto issue = [&](int st) {
+        const int64_t base = (int64_t)(st >> 1) * kChunk + (st & 1) * 2048;
+        char *lx = smem[st % kNbuf][0];
+        char *ly = smem[st % kNbuf][1];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int seg = wave * 4 + t;
+            const int64_t off = base + (seg >> 1) * 4096 + (seg & 1) * 1024 + lane * 16;
+            gl_lds16(xs + off, lx + seg * 1024);
+            gl_lds16(ys + off, ly + seg * 1024);
+        }
+    };
+    auto frags = [&](intx8(&xd)[4], intx8(&yd)[4], int st) {
+        const char *lx = smem[st % kNbuf][0];
+        const char *ly = smem[st % kNbuf][1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int fx = (wm * 4 + q) * 2048 + lane * 16;
+            const int fy = (wn * 4 + q) * 2048 + lane * 16;
+            const intx4 xlo = *reinterpret_cast<const intx4 *>(lx + fx);
+            const intx4 xhi = *reinterpret_cast<const intx4 *>(lx + fx + 1024);
+            const intx4 ylo = *reinterpret_cast<const intx4 *>(ly + fy);
+            const intx4 yhi = *reinterpret_cast<const intx4 *>(ly + fy + 1024);
+            xd[q] = intx8{xlo[0], xlo[1], xlo[2], xlo[3], xhi[0], xhi[1], xhi[2], xhi[3]};
+            yd[q] = intx8{ylo[0], ylo[1], ylo[2], ylo[3], yhi[0], yhi[1], yhi[2], yhi[3]};
+        }
+    };
 
     floatx16 acc[4][4];
 #pragma unroll
@@ -169,48 +319,56 @@ k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = floatx16{};
 
-    load_chunk(xs, smem[0][0], wave, lane);
-    load_chunk(ys, smem[0][1], wave, lane);
-    wait_vmem();
+    for (int st = 0; st < kNbuf - 1 && st < nks; ++st) issue(st);
+    wait_stages(min(kNbuf - 2, nks - 1));
     __syncthreads();
-    for (int64_t ks = 0; ks < nks; ++ks) {
-        const int buf = (int)(ks & 1);
-        if (ks + 1 < nks) {
-            load_chunk(xs + (ks + 1) * kChunk, smem[buf ^ 1][0], wave, lane);
-            load_chunk(ys + (ks + 1) * kChunk, smem[buf ^ 1][1], wave, lane);
-        }
-        // E8M0 scales: value = fp8 * 2^(s - 127) with s = 127 - e
-        const int kt = (int)((ks * OMX_KSTAGE8) / OMX_SCALE_TILE);
+    intx8 xa[4], ya[4], xb[4], yb[4];
+    frags(xa, ya, 0);
+    // E8M0 scales: value = fp8 * 2^(s - 127) with s = 127 - e
+    auto body = [&](intx8(&xc)[4], intx8(&yc)[4], intx8(&xn)[4], intx8(&yn)[4], int cs) {
+        const int kt = (cs * 64) / OMX_SCALE_TILE;
         const int sx = 127 - xe[kt];
         const int sy = 127 - ye[kt];
-        const char *xl = smem[buf][0];
-        const char *yl = smem[buf][1];
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int kk = 0; kk < 2; ++kk) {
-            intx8 xf[4], yf[4];
+        for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int fx = ((wm * 4 + q) * 2 + kk) * 2048 + lane * 16;
-                const int fy = ((wn * 4 + q) * 2 + kk) * 2048 + lane * 16;
-                const intx4 xlo = *reinterpret_cast<const intx4 *>(xl + fx);
-                const intx4 xhi = *reinterpret_cast<const intx4 *>(xl + fx + 1024);
-                const intx4 ylo = *reinterpret_cast<const intx4 *>(yl + fy);
-                const intx4 yhi = *reinterpret_cast<const intx4 *>(yl + fy + 1024);
-                xf[q] = intx8{xlo[0], xlo[1], xlo[2], xlo[3], xhi[0], xhi[1], xhi[2], xhi[3]};
-                yf[q] = intx8{ylo[0], ylo[1], ylo[2], ylo[3], yhi[0], yhi[1], yhi[2], yhi[3]};
-            }
-#pragma unroll
-            for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-                for (int ni = 0; ni < 4; ++ni)
-                    acc[mi][ni] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                        yf[ni], xf[mi], acc[mi][ni], 0, 0, 0, sy, 0, sx);
+            for (int ni = 0; ni < 4; ++ni)
+                acc[mi][ni] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(yc[ni], xc[mi], acc[mi][ni], 0, 0, 0,
+                                                                              sy, 0, sx);
+        __builtin_amdgcn_sched_barrier(0);
+        const bool more = cs + 1 < nks;
+        if (more) {
+            wait_stages(min(kNbuf - 3, nks - cs - 2));
+            __syncthreads();
+            if (cs + kNbuf - 1 < nks) issue(cs + kNbuf - 1);
         }
-        wait_vmem();
-        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        if (more) frags(xn, yn, cs + 1);
+#pragma unroll
+        for (int mi = 2; mi < 4; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < 4; ++ni)
+                acc[mi][ni] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(yc[ni], xc[mi], acc[mi][ni], 0, 0, 0,
+                                                                              sy, 0, sx);
+        if (more) {
+#pragma unroll
+            for (int g = 0; g < 8; ++g) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
+                __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // 2 DS reads
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    // nks is a multiple of 4 (k % 256 == 0): stages come in pairs
+    for (int st = 0; st < nks; st += 2) {
+        body(xa, ya, xb, yb, st);
+        body(xb, yb, xa, ya, st + 1);
     }
     epilogue<kBetaZero>(acc, M, N, alpha, beta, C, ldc, tm, tn, wm, wn, lane);
 }
+
+constexpr int kNbuf = OMX_GEMM_NBUF;
 
 int gemm_args(int64_t m, int64_t n, int64_t k, int64_t kstage, const void *X, int64_t xr0,
               const void *Y, int64_t yr0, const float *C, int64_t ldc) {
@@ -236,10 +394,10 @@ int omx_gemm_f16_packed(int64_t m, int64_t n, int64_t k, float alpha, const uint
     const char *yp = reinterpret_cast<const char *>(Y) + (y_row0 / OMX_STRIP) * OMX_STRIP * k * 2;
     dim3 grid((unsigned)(tiles_m * tiles_n));
     if (beta == 0.0f)
-        hipLaunchKernelGGL(k_gemm_f16<true>, grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
+        hipLaunchKernelGGL((k_gemm_f16<true, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
                            beta, xp, yp, C, ldc, tiles_m, tiles_n);
     else
-        hipLaunchKernelGGL(k_gemm_f16<false>, grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
+        hipLaunchKernelGGL((k_gemm_f16<false, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
                            beta, xp, yp, C, ldc, tiles_m, tiles_n);
     OMX_LAUNCH_CHECK();
     return OMX_OK;
@@ -260,10 +418,10 @@ int omx_gemm_f8_packed(int64_t m, int64_t n, int64_t k, float alpha, const uint8
     const int32_t *ye = yexp + (y_row0 / OMX_STRIP) * KT;
     dim3 grid((unsigned)(tiles_m * tiles_n));
     if (beta == 0.0f)
-        hipLaunchKernelGGL(k_gemm_f8<true>, grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
+        hipLaunchKernelGGL((k_gemm_f8<true, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
                            beta, xp, xe, yp, ye, C, ldc, tiles_m, tiles_n);
     else
-        hipLaunchKernelGGL(k_gemm_f8<false>, grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
+        hipLaunchKernelGGL((k_gemm_f8<false, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
                            beta, xp, xe, yp, ye, C, ldc, tiles_m, tiles_n);
     OMX_LAUNCH_CHECK();
     return OMX_OK;
