@@ -9,12 +9,18 @@ reference, main.cpp:529-554/:621), then the no-pivot LU with look-ahead and
 the FP64 refinement run inside a timed region bracketed by device
 synchronisations.  Throughput is (2/3 n^3 + 3/2 n^2) / time
 (main.cpp:663).  Each step is timed separately because the factorisation
-overwrites the matrix; `value` is steps * flops / sum of step times.
+overwrites the matrix; a rate is steps * flops / sum of step times.
 
-The line also carries the Schur GEMM roofline (trailing-update launches,
-timed with HIP events on the stream they run on), the FP8 e4m3 schedule
-measured the same way, the forward error against an FP64 refinement to the
-floor, and the CPU oracle timed on this host as the baseline.
+Two methods are measured on the same system:
+  plain    FP16 Schur update, classical IR (the reference's method);
+  learned  the MLP policy (openmxp_amd/data/sched_mlp.f32) picks FP8 or FP16
+           per LU step, GMRES-IR; if the refinement fails the system is
+           generated again and factored in FP16, inside the timed region.
+`value` is the learned method's rate; the line carries the plain rate and
+the ratio.  It also carries the Schur GEMM roofline of the plain run (the
+trailing-update launches, timed with HIP events on the stream they run on),
+the forward errors against an FP64 refinement to the floor, and the CPU
+oracle timed on this host as the baseline.
 """
 from __future__ import annotations
 
@@ -22,6 +28,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -30,6 +37,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP16_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
 PEAK_FP8_TFLOPS = 5000.0   # MI355X dense FP8 MFMA
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "gemm_f16_traffic_n204800.txt")
 
 
 def hpl_flops(n: int) -> float:
@@ -45,7 +53,8 @@ def parse():
     ap.add_argument("--b", type=int, default=2560)
     ap.add_argument("--alt", type=int, default=1)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--no-fp8", action="store_true", help="skip the FP8 schedule measurement")
+    ap.add_argument("--no-learned", action="store_true", help="measure the plain method only")
+    ap.add_argument("--fp8-all", action="store_true", help="also measure FP8 on every step")
     ap.add_argument("--cpu-n", type=int, default=10240, help="order of the CPU baseline sample")
     ap.add_argument("--cpu-b", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
@@ -57,7 +66,8 @@ def log(msg):
 
 
 class GpuSolve:
-    def __init__(self, n, b, alt, fp8):
+    def __init__(self, n, b, alt, fp8, policy=None):
+        import numpy as np
         import torch
         from openmxp_amd import _lib
         self.torch = torch
@@ -72,6 +82,9 @@ class GpuSolve:
         rc = self.L.omx_solver_create(n, b, alt, 1 if fp8 else 0, C.byref(self.h))
         if rc != 0:
             raise RuntimeError(f"omx_solver_create failed: {rc}")
+        if policy is not None:
+            self.params = np.ascontiguousarray(policy, dtype=np.float32)
+            assert self.L.omx_solver_set_policy(self.h, self.params.ctypes.data) == 0
 
     def generate(self, seed):
         L = self.L
@@ -79,10 +92,8 @@ class GpuSolve:
         assert L.omx_rhs_f64(self.n, seed, self.rhs.data_ptr(), None) == 0
         self.torch.cuda.synchronize()
 
-    def solve(self, seed, sched=None, method=0):
-        """One timed LU + IR; returns dict with seconds, LU timings, IR results."""
+    def _lu_ir(self, seed, sched, method):
         import numpy as np
-        torch = self.torch
         tim = np.zeros(5)
         out = np.zeros(4)
         trace = np.zeros(100)
@@ -90,30 +101,45 @@ class GpuSolve:
         if sched is not None:
             s = np.ascontiguousarray(sched, dtype=np.int32)
             sp = s.ctypes.data
-        torch.cuda.synchronize()
         t0 = time.perf_counter()
         info = self.L.omx_lu_factor(self.h, self.A.data_ptr(), self.lda, sp, tim.ctypes.data)
         t1 = time.perf_counter()
         rc = self.L.omx_ir_solve(self.h, self.A.data_ptr(), self.lda, seed, self.d.data_ptr(), self.rhs.data_ptr(),
                                  self.x.data_ptr(), method, 50, 50, 1e-10, 16.0, 0, out.ctypes.data,
                                  trace.ctypes.data)
-        torch.cuda.synchronize()
+        self.torch.cuda.synchronize()
         t2 = time.perf_counter()
-        return dict(seconds=t2 - t0, lu_s=t1 - t0, ir_s=t2 - t1, info=int(info), ir_rc=int(rc),
-                    gemm_ms=tim[1], gemm_launches=int(tim[2]), gemm_flops=tim[3], resid=out[0],
-                    harness=out[1], ir_steps=int(out[2]), inner=int(out[3]))
+        return info, rc, tim, out, t1 - t0, t2 - t1
+
+    def solve(self, seed, sched=None, method=0, fallback=False):
+        """One timed LU + IR (+ the FP16 fallback when asked and needed)."""
+        import numpy as np
+        self.torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info, rc, tim, out, lu_s, ir_s = self._lu_ir(seed, sched, method)
+        used = np.zeros(self.n // self.b, np.int32)
+        self.L.omx_solver_last_schedule(self.h, used.ctypes.data, None)
+        fell_back = False
+        if fallback and (rc != 0 or info != 0 or not out[1] < 16.0):
+            fell_back = True
+            self.generate(seed)
+            info, rc, tim, out, lu_s, ir_s = self._lu_ir(seed, np.zeros(self.n // self.b, np.int32), method)
+        t = time.perf_counter() - t0
+        return dict(seconds=t, lu_s=lu_s, ir_s=ir_s, info=int(info), ir_rc=int(rc), gemm_ms=tim[1],
+                    gemm_launches=int(tim[2]), gemm_flops=tim[3], resid=out[0], harness=out[1],
+                    ir_steps=int(out[2]), inner=int(out[3]), fp8_steps=int((used[:-1] > 0).sum()),
+                    steps_total=int(used.size - 1), fell_back=fell_back)
 
     def forward_error(self, seed):
         """FP64 refinement of the current solution to the floor (3 more
         corrections, no stopping test): the large-n FP64 oracle."""
         import numpy as np
-        torch = self.torch
         xs = self.x.clone()
         out = np.zeros(4)
         trace = np.zeros(8)
         rc = self.L.omx_ir_solve(self.h, self.A.data_ptr(), self.lda, seed, self.d.data_ptr(), self.rhs.data_ptr(),
                                  self.x.data_ptr(), 0, 4, 50, 1e-10, 0.0, 1, out.ctypes.data, trace.ctypes.data)
-        torch.cuda.synchronize()
+        self.torch.cuda.synchronize()
         if rc != 0:
             return None
         fwd = float((xs - self.x).abs().max() / self.x.abs().max())
@@ -122,23 +148,39 @@ class GpuSolve:
 
     def close(self):
         self.L.omx_solver_destroy(self.h)
+        del self.A
 
 
-def measure(gs, args, sched, label):
+def measure(gs, args, label, sched=None, method=0, fallback=False):
     res = []
     for w in range(args.warmup):
         gs.generate(args.seed)
-        r = gs.solve(args.seed, sched)
+        r = gs.solve(args.seed, sched, method, fallback)
         log(f"[{label}] warmup {w}: {r['seconds']:.3f} s harness {r['harness']:.4f}")
     for k in range(args.steps):
         gs.generate(args.seed)
-        r = gs.solve(args.seed, sched)
+        r = gs.solve(args.seed, sched, method, fallback)
         log(f"[{label}] step {k}: {r['seconds']:.3f} s (LU {r['lu_s']:.3f}, IR {r['ir_s']:.3f}, "
-            f"{r['ir_steps']} corrections) harness {r['harness']:.4f}, GEMM "
-            f"{r['gemm_flops'] / (r['gemm_ms'] * 1e-3) * 1e-12 if r['gemm_ms'] else 0:.1f} TFLOP/s")
+            f"{r['ir_steps']} corrections, {r['inner']} inner) harness {r['harness']:.4f}, FP8 steps "
+            f"{r['fp8_steps']}/{r['steps_total']}, trailing GEMM "
+            f"{r['gemm_flops'] / (r['gemm_ms'] * 1e-3) * 1e-12 if r['gemm_ms'] else 0:.1f} TFLOP/s"
+            + (" [fell back to FP16]" if r["fell_back"] else ""))
         res.append(r)
     fwd = gs.forward_error(args.seed)
     return res, fwd
+
+
+def agg(res, flops):
+    t = sum(r["seconds"] for r in res)
+    gm = sum(r["gemm_ms"] for r in res)
+    gf = sum(r["gemm_flops"] for r in res)
+    gl = sum(r["gemm_launches"] for r in res)
+    ok = all(r["ir_rc"] == 0 and r["harness"] < 16.0 and r["info"] == 0 for r in res)
+    last = res[-1]
+    return dict(tflops=len(res) * flops / t * 1e-12, ms=t / len(res) * 1e3, gemm_ms=gm, gemm_flops=gf,
+                gemm_launches=gl, ok=ok, harness=last["harness"], resid=last["resid"], ir_steps=last["ir_steps"],
+                inner=last["inner"], lu_s=last["lu_s"], ir_s=last["ir_s"], fp8_steps=last["fp8_steps"],
+                steps_total=last["steps_total"], fell_back=any(r["fell_back"] for r in res))
 
 
 def cpu_baseline(n, b, alt, seed):
@@ -154,73 +196,100 @@ def cpu_baseline(n, b, alt, seed):
                 iters=res["iters"], info=info)
 
 
+def pmc_traffic():
+    """HBM bytes per launch of the FP16 Schur GEMM from the committed PMC
+    summary (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated), if any."""
+    try:
+        txt = open(PMC_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
-                          "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         log("bench.py measures one MI355X: multi-GPU runs are not part of this build")
         sys.exit(2)
     n, b = args.n, args.b
     flops = hpl_flops(n)
+    policy = None
+    if not args.no_learned:
+        from openmxp_amd import scheduler
+        try:
+            policy = scheduler.load_params()
+        except (OSError, ValueError) as e:
+            log(f"no trained policy ({e}); measuring the plain method only")
     log(f"n={n} b={b} alt={args.alt}: generating and solving on cuda:0")
-    gs = GpuSolve(n, b, args.alt, fp8=not args.no_fp8)
-    res16, fwd16 = measure(gs, args, None, "fp16")
-    res8, fwd8 = (None, None)
-    if not args.no_fp8:
-        res8, fwd8 = measure(gs, args, [1] * (n // b), "fp8")
+    gs = GpuSolve(n, b, args.alt, fp8=False)
+    res16, fwd16 = measure(gs, args, "plain fp16 + IR")
     gs.close()
-
-    def agg(res):
-        t = sum(r["seconds"] for r in res)
-        gm = sum(r["gemm_ms"] for r in res)
-        gf = sum(r["gemm_flops"] for r in res)
-        gl = sum(r["gemm_launches"] for r in res)
-        ok = all(r["ir_rc"] == 0 and r["harness"] < 16.0 and r["info"] == 0 for r in res)
-        return dict(tflops=len(res) * flops / t * 1e-12, ms=t / len(res) * 1e3, gemm_ms=gm, gemm_flops=gf,
-                    gemm_launches=gl, ok=ok, harness=res[-1]["harness"], resid=res[-1]["resid"],
-                    ir_steps=res[-1]["ir_steps"], lu_s=res[-1]["lu_s"], ir_s=res[-1]["ir_s"])
-
-    a16 = agg(res16)
-    a8 = agg(res8) if res8 else None
-    achieved = a16["gemm_flops"] / a16["gemm_launches"] / (a16["gemm_ms"] * 1e-3 / a16["gemm_launches"]) * 1e-12
+    del gs
+    a16 = agg(res16, flops)
+    aL = a8 = None
+    fwdL = fwd8 = None
+    if policy is not None or args.fp8_all:
+        gs = GpuSolve(n, b, args.alt, fp8=True, policy=policy)
+        if policy is not None:
+            resL, fwdL = measure(gs, args, "learned + GMRES-IR", None, 1, True)
+            aL = agg(resL, flops)
+        if args.fp8_all:
+            res8, fwd8 = measure(gs, args, "fp8 + GMRES-IR", [1] * (n // b), 1)
+            a8 = agg(res8, flops)
+        gs.close()
+        del gs
+    achieved = a16["gemm_flops"] / (a16["gemm_ms"] * 1e-3) * 1e-12
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline: oracle solve at n={args.cpu_n} b={args.cpu_b}")
         cpu = cpu_baseline(args.cpu_n, args.cpu_b, args.alt, args.seed)
+    head = aL if aL else a16
     line = {
         "metric": "LU+IR TFLOP/s @N=204800 1 GPU, learned FP8/FP16 vs plain FP16; resid<16",
-        "value": round(a16["tflops"], 3),
+        "value": round(head["tflops"], 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(a16["ms"], 2),
+        "ms_per_step": round(head["ms"], 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16 Schur (fp32 acc) / fp32 panels / fp64 IR",
+        "dtype": ("fp8/fp16 Schur (fp32 acc) per learned step / fp32 panels / fp64 GMRES-IR" if aL else
+                  "fp16 Schur (fp32 acc) / fp32 panels / fp64 IR"),
         "data": "synthetic: reference LCG generator, seed %d, diagonally dominant" % args.seed,
-        "config": {"workload": f"HPL-MxP LU+IR n={n} b={b} 1x1 grid alt={args.alt} classical IR",
-                   "n": n, "b": b, "schedule": "plain fp16 (learned schedule not built yet)"},
-        "valid": a16["ok"],
-        "harness": a16["harness"],
-        "harness_eps": a16["harness"] / 2.0,
-        "resid": a16["resid"],
-        "ir_steps": a16["ir_steps"],
-        "forward_error": fwd16,
-        "lu_s": round(a16["lu_s"], 4),
-        "ir_s": round(a16["ir_s"], 4),
+        "config": {"workload": f"HPL-MxP LU+IR n={n} b={b} 1x1 grid alt={args.alt}", "n": n, "b": b,
+                   "method": ("learned per-step FP8/FP16 schedule + GMRES-IR(50), FP16 fallback" if aL else
+                              "plain FP16 + classical IR")},
+        "valid": head["ok"],
+        "harness": head["harness"],
+        "harness_eps": head["harness"] / 2.0,
+        "resid": head["resid"],
+        "ir_steps": head["ir_steps"],
+        "forward_error": fwdL if aL else fwd16,
+        "lu_s": round(head["lu_s"], 4),
+        "ir_s": round(head["ir_s"], 4),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_FP16_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP16_TFLOPS, 4), "traffic": None,
-                     "kernel": "k_gemm_f16 trailing update (alpha=-1, beta=1)",
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP16_TFLOPS, 4), "traffic": pmc_traffic(),
+                     "kernel": "k_gemm_f16 trailing update (alpha=-1, beta=1), plain run",
                      "launches": a16["gemm_launches"]},
+        "plain_fp16": {"tflops": round(a16["tflops"], 3), "ms_per_step": round(a16["ms"], 2), "valid": a16["ok"],
+                       "harness": a16["harness"], "ir_steps": a16["ir_steps"], "forward_error": fwd16,
+                       "lu_s": round(a16["lu_s"], 4), "ir_s": round(a16["ir_s"], 4)},
     }
+    if aL:
+        achL = aL["gemm_flops"] / (aL["gemm_ms"] * 1e-3) * 1e-12
+        line["learned"] = {"speedup_vs_plain_fp16": round(aL["tflops"] / a16["tflops"], 3),
+                           "fp8_steps": aL["fp8_steps"], "steps": aL["steps_total"], "inner_iters": aL["inner"],
+                           "fell_back_to_fp16": aL["fell_back"], "trailing_gemm_tflops": round(achL, 1),
+                           "trailing_gemm_frac_fp8_peak": round(achL / PEAK_FP8_TFLOPS, 4)}
     if a8:
         ach8 = a8["gemm_flops"] / (a8["gemm_ms"] * 1e-3) * 1e-12
         line["fp8_all"] = {"tflops": round(a8["tflops"], 3), "ms_per_step": round(a8["ms"], 2), "valid": a8["ok"],
                            "harness": a8["harness"], "ir_steps": a8["ir_steps"], "forward_error": fwd8,
-                           "speedup_vs_fp16": round(a8["tflops"] / a16["tflops"], 3),
+                           "speedup_vs_plain_fp16": round(a8["tflops"] / a16["tflops"], 3),
                            "gemm_tflops": round(ach8, 1), "gemm_frac_fp8_peak": round(ach8 / PEAK_FP8_TFLOPS, 4)}
     if cpu:
         line["cpu_baseline"] = {"value": round(cpu["value"], 5), "unit": "TFLOP/s", "cores": cpu["threads"],

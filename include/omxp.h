@@ -113,21 +113,66 @@ int omx_div_f64(int64_t n, const double *b, const double *d, double *x,
 int omx_axpy_f64(int64_t n, double alpha, const double *x, double *y,
                  void *stream);
 
-/* Bytes of scratch the blocked triangular solves need for order n. */
+/* ---------------------------------------------------------------------------
+ * Conditioned test matrices ("randsvd"): A = U diag(sigma) V^T with
+ * sigma_i = kappa^(-i/(n-1)), U and V products of nrefl seeded Householder
+ * reflectors each (reflector t's vector: the LCG values at linear indices
+ * n^2 + n + t*n + i, normalised).  Held as A = diag(sigma) + P Q^T with
+ * P, Q of n x rank, rank = 2*nrefl <= 16, column-major.  No reference
+ * counterpart (the reference's Higham generator, highammgen.hpp:4-30, is a
+ * different family and unreachable from its driver).
+ * ------------------------------------------------------------------------- */
+
+/* sigma (n), P, Q (n x 2*nrefl) into HOST arrays (FP64). */
+int omx_randsvd_factors_host(int64_t n, uint64_t seed, double kappa,
+                             int nrefl, double *sigma, double *P, double *Q);
+
+/* As omx_randsvd_factors_host into DEVICE arrays (computed on the host,
+ * copied on `stream`; returns after the copies). */
+int omx_randsvd_factors(int64_t n, uint64_t seed, double kappa, int nrefl,
+                        double *sigma, double *P, double *Q, void *stream);
+
+/* Bytes of scratch omx_randsvd_f32 needs for order n. */
+size_t omx_randsvd_workspace_bytes(int64_t n);
+
+/* A (FP32, RNE of the FP64 entries) = diag(sigma) + P Q^T; diag (FP64, n)
+ * receives A_ii and rowabs (FP64, n) the row sums of |A_ij| (their max is
+ * ||A||_inf). */
+int omx_randsvd_f32(int64_t n, int64_t rank, const double *sigma,
+                    const double *P, const double *Q, float *A, int64_t lda,
+                    double *diag, double *rowabs, void *work,
+                    size_t work_bytes, void *stream);
+
+/* Bytes of scratch omx_lowrank_gemv_f64 needs. */
+size_t omx_lowrank_gemv_workspace_bytes(int64_t rank);
+
+/* y += alpha * (diag(sigma) + P Q^T) x in FP64: the residual product of the
+ * refinement for a randsvd system (O(n * rank) work). */
+int omx_lowrank_gemv_f64(int64_t n, int64_t rank, const double *sigma,
+                         const double *P, const double *Q, double alpha,
+                         const double *x, double *y, void *work,
+                         size_t work_bytes, void *stream);
+
+/* Bytes of scratch the triangular solves need for order n: 0 (they work in
+ * place on x); kept so callers written against a workspace-taking solve
+ * keep working.  `work` may be NULL. */
 size_t omx_trsv_workspace_bytes(int64_t n);
 
-/* x <- L^-1 x with L the unit-lower FP32 factor stored in A (order n, block
- * size b, n % b == 0), FP64 arithmetic: per block column the accumulated
- * contributions are subtracted, the diagonal block is solved, and the
- * blocks below gain L_ij x_j.  Replaces trsvL_h (panel_trsv.hpp:308-449:
- * cast_copy_general + rocblas_dtrsv + HPLMXP_gemv + daxpy) for one GPU,
- * without the reference's per-block-column swap of accumulation buffers. */
+/* x <- L^-1 x with L the unit-lower FP32 factor stored in A (order n), FP64
+ * arithmetic.  `b` is the factorisation's block size and only checked
+ * (n % b == 0); the solve walks 256-column blocks: per block column the
+ * next diagonal block is updated and solved while the rows below receive
+ * L_ij x_j, one launch per block column.  Replaces trsvL_h
+ * (panel_trsv.hpp:308-449: cast_copy_general + rocblas_dtrsv +
+ * HPLMXP_gemv + daxpy) for one GPU, without the reference's per-block-column
+ * swap of accumulation buffers. */
 int omx_trsv_lower_unit_f32f64(int64_t n, int64_t b, const float *A,
                                int64_t lda, double *x, void *work,
                                size_t work_bytes, void *stream);
 
-/* x <- U^-1 x with U the upper non-unit FP32 factor in A.  Replaces trsvU_h
- * (panel_trsv.hpp:452-597) for one GPU. */
+/* x <- U^-1 x with U the upper non-unit FP32 factor in A, block columns
+ * from the last to the first.  Replaces trsvU_h (panel_trsv.hpp:452-597)
+ * for one GPU. */
 int omx_trsv_upper_f32f64(int64_t n, int64_t b, const float *A, int64_t lda,
                           double *x, void *work, size_t work_bytes,
                           void *stream);
@@ -170,6 +215,21 @@ int omx_pack_f16(int64_t rows, int64_t k, const float *src, int64_t rs,
 int omx_pack_f8(int64_t rows, int64_t k, const float *src, int64_t rs,
                 int64_t ls, uint8_t *P, int32_t *exps, float *tile_max,
                 void *stream);
+
+/* omx_pack_f8 with `headroom` (0..3) bits below the tile maximum: e is the
+ * largest with max|tile| * 2^e <= 448 * 2^-headroom.  have_max = 1: tile_max
+ * already holds the tile maxima (from omx_panel_tile_maxabs) and is not
+ * recomputed.  No reference counterpart. */
+int omx_pack_f8_ex(int64_t rows, int64_t k, const float *src, int64_t rs,
+                   int64_t ls, int headroom, int have_max, uint8_t *P,
+                   int32_t *exps, float *tile_max, void *stream);
+
+/* max |S| of each 256 x 256 tile of the panel S (k % 256 == 0),
+ * omx_panel_exps_count(rows, k) floats, strip major.  No reference
+ * counterpart (feeds the FP8 scales and the precision scheduler). */
+int omx_panel_tile_maxabs(int64_t rows, int64_t k, const float *src,
+                          int64_t rs, int64_t ls, float *tile_max,
+                          void *stream);
 
 /* C = beta*C + alpha * X(x_row0 : x_row0+m, :) * Y(y_row0 : y_row0+n, :)^T
  * with packed FP16 panels X, Y of k columns (k % 64 == 0), FP32
@@ -229,6 +289,51 @@ int omx_copy_block_f32(int64_t m, int64_t n, const float *A, int64_t lda,
                        float *B, int64_t ldb, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Learned precision schedule: an MLP 16-64-64-3 (FP32, ReLU) that picks, per
+ * LU step, FP16 or FP8 for that step's Schur panels (logits z0, z1) and the
+ * FP8 tile-scale headroom (z2, bits, rounded and clamped to 0..3).  No
+ * reference counterpart.  Parameters: one FP32 array of OMX_MLP_NPARAMS,
+ *   W1[16][64] | b1[64] | W2[64][64] | b2[64] | W3[64][3] | b3[3]
+ * (W[i][j]: input i -> unit j).  Loss per sample: softmax cross-entropy of
+ * (z0, z1) against the class y (0 FP16, 1 FP8) + 0.5 m (z2 - t)^2; the
+ * batch loss is the mean.
+ * ------------------------------------------------------------------------- */
+#define OMX_MLP_NPARAMS 5443
+#define OMX_MLP_NFEATURES 16
+
+/* H1, H2 (batch x 64) and Z (batch x 3) for the inputs X (batch x 16),
+ * all row-major. */
+int omx_mlp_forward_f32(int64_t batch, const float *params, const float *X,
+                        float *H1, float *H2, float *Z, void *stream);
+
+/* Bytes of scratch omx_mlp_backward_f32 needs. */
+size_t omx_mlp_backward_workspace_bytes(int64_t batch);
+
+/* grad (OMX_MLP_NPARAMS) and *loss (device float) of the mean loss, from a
+ * forward pass's H1, H2, Z; y (int32), t, m (float) per sample. */
+int omx_mlp_backward_f32(int64_t batch, const float *params, const float *X,
+                         const float *H1, const float *H2, const float *Z,
+                         const int32_t *y, const float *t, const float *m,
+                         float *grad, float *loss, void *work,
+                         size_t work_bytes, void *stream);
+
+/* One Adam step (bias-corrected, step >= 1) on params with moments mom, vel. */
+int omx_mlp_adam_f32(int64_t nparams, float *params, const float *grad,
+                     float *mom, float *vel, float lr, float beta1,
+                     float beta2, float eps, int step, void *stream);
+
+/* The 16 scheduler inputs of LU step `step` (features, device): log2
+ * statistics (max, min, mean, spread) of the column and row panels' tile
+ * maxima, log2 max/min |u_ii| of the factored diagonal block Akk (order b),
+ * the step's position, the trailing size, the panels' size relative to the
+ * pivots, and the previous step's choice `prev`. */
+int omx_sched_features_f32(int64_t ntiles_l, const float *lmax,
+                           int64_t ntiles_r, const float *rmax,
+                           const float *Akk, int64_t lda, int64_t b,
+                           int64_t step, int64_t nb, int64_t n, int prev,
+                           float *features, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Solver: the blocked LU with look-ahead and the FP64 refinement, driven by
  * the host.  A handle owns two HIP streams, events and the panel and vector
  * workspaces (allocated once by omx_solver_create; the factorisation and
@@ -251,14 +356,27 @@ int omx_solver_destroy(omx_solver *s);
  * lda % 4 == 0) in the step order of panel_lu_async_wGPU_gdirect_alt2
  * (panel_trf_gdirect.hpp:593-1192) on one GPU, with the diagonal/panel work
  * of step k overlapped with the trailing update of step k on a second
- * stream.  sched (host, n/b ints, optional): 1 packs step k's Schur panels
- * in scaled FP8 e4m3 (requires OMX_SOLVER_FP8), 0 in FP16.  Blocks until
- * done.  timings (host, optional, 5 doubles): wall ms, summed ms of the
+ * stream.  sched (host, n/b ints, optional) fixes each step's Schur panels:
+ * 0 FP16, 1 + h scaled FP8 e4m3 with tile-scale headroom h (0..3; FP8
+ * requires OMX_SOLVER_FP8).  sched = NULL: the learned policy if one was set
+ * (omx_solver_set_policy), otherwise FP16 throughout.  Blocks until done.
+ * timings (host, optional, 5 doubles): wall ms, summed ms of the
  * trailing-update GEMMs (HIP events on their stream), their count, their
  * flops, flops of the panel GEMMs.  Returns 0 or the 1-based index of the
  * first zero pivot. */
 int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched,
                   double *timings);
+
+/* Set (params: host array of OMX_MLP_NPARAMS floats) or clear (NULL) the
+ * learned schedule used by omx_lu_factor when sched is NULL.  Needs a solver
+ * created with OMX_SOLVER_FP8. */
+int omx_solver_set_policy(omx_solver *s, const float *params);
+
+/* The last factorisation's per-step choices (sched, n/b ints: 0 FP16,
+ * 1 + h FP8) and, for FP8 solvers, the 16 scheduler features of every step
+ * (features, n/b x 16 floats, row-major; zero for the last step, which has
+ * no panels).  Either pointer may be NULL. */
+int omx_solver_last_schedule(omx_solver *s, int *sched, float *features);
 
 /* FP64 refinement of A x = rhs with the FP32 factors LU (device) and the
  * generated A (regenerated from seed; diag = its FP64 diagonal).
@@ -279,6 +397,31 @@ int omx_ir_solve(omx_solver *s, const float *LU, int64_t lda, uint64_t seed,
                  int method, int maxit, int restart, double inner_tol,
                  double stop_harness, int x_given, double *out,
                  double *trace);
+
+/* The system a refinement solves.  kind OMX_SYSTEM_LCG: the reference
+ * generator (seed; ||A||_inf = 2 max diag, panel_norm.hpp:129-135).
+ * kind OMX_SYSTEM_LOWRANK: diag(sigma) + P Q^T (rank, sigma, P, Q device;
+ * norm_inf = ||A||_inf given by the caller).  diag (device, n) is A's FP64
+ * diagonal for both kinds (x0 = rhs ./ diag). */
+#define OMX_SYSTEM_LCG 0
+#define OMX_SYSTEM_LOWRANK 1
+typedef struct omx_system {
+    int kind;
+    uint64_t seed;
+    const double *diag;
+    double norm_inf;
+    int64_t rank;
+    const double *sigma;
+    const double *P;
+    const double *Q;
+} omx_system;
+
+/* omx_ir_solve for any omx_system (same arguments and results). */
+int omx_ir_solve_system(omx_solver *s, const float *LU, int64_t lda,
+                        const omx_system *sys, const double *rhs, double *x,
+                        int method, int maxit, int restart, double inner_tol,
+                        double stop_harness, int x_given, double *out,
+                        double *trace);
 
 /* hpl-harness ||r||/(||A|| ||x|| + ||b||)/(n * DBL_EPSILON/2) on the host
  * (iterative_refinement.hpp:154-157). */

@@ -33,6 +33,12 @@ SIGNATURES = {
     "omx_copy_f64": (i32, [i64, vp, vp, vp]),
     "omx_div_f64": (i32, [i64, vp, vp, vp, vp]),
     "omx_axpy_f64": (i32, [i64, dbl, vp, vp, vp]),
+    "omx_randsvd_factors_host": (i32, [i64, u64, dbl, i32, vp, vp, vp]),
+    "omx_randsvd_factors": (i32, [i64, u64, dbl, i32, vp, vp, vp, vp]),
+    "omx_randsvd_workspace_bytes": (sz, [i64]),
+    "omx_randsvd_f32": (i32, [i64, i64, vp, vp, vp, vp, i64, vp, vp, vp, sz, vp]),
+    "omx_lowrank_gemv_workspace_bytes": (sz, [i64]),
+    "omx_lowrank_gemv_f64": (i32, [i64, i64, vp, vp, vp, dbl, vp, vp, vp, sz, vp]),
     "omx_trsv_workspace_bytes": (sz, [i64]),
     "omx_trsv_lower_unit_f32f64": (i32, [i64, i64, vp, i64, vp, vp, sz, vp]),
     "omx_trsv_upper_f32f64": (i32, [i64, i64, vp, i64, vp, vp, sz, vp]),
@@ -40,6 +46,8 @@ SIGNATURES = {
     "omx_panel_exps_count": (sz, [i64, i64]),
     "omx_pack_f16": (i32, [i64, i64, vp, i64, i64, i32, vp, vp, vp]),
     "omx_pack_f8": (i32, [i64, i64, vp, i64, i64, vp, vp, vp, vp]),
+    "omx_pack_f8_ex": (i32, [i64, i64, vp, i64, i64, i32, i32, vp, vp, vp, vp]),
+    "omx_panel_tile_maxabs": (i32, [i64, i64, vp, i64, i64, vp, vp]),
     "omx_gemm_f16_packed": (i32, [i64, i64, i64, flt, vp, i64, vp, i64, flt, vp, i64, vp]),
     "omx_gemm_f8_packed": (i32, [i64, i64, i64, flt, vp, vp, i64, vp, vp, i64, flt, vp, i64, vp]),
     "omx_sgemm_f32": (i32, [i64, i64, i64, flt, vp, i64, vp, i64, flt, vp, i64, vp]),
@@ -47,12 +55,31 @@ SIGNATURES = {
     "omx_trsm_f32": (i32, [i32, i64, i64, vp, i64, vp, i64, vp]),
     "omx_identity_f32": (i32, [i64, vp, i64, vp]),
     "omx_copy_block_f32": (i32, [i64, i64, vp, i64, vp, i64, vp]),
+    "omx_mlp_forward_f32": (i32, [i64, vp, vp, vp, vp, vp, vp]),
+    "omx_mlp_backward_workspace_bytes": (sz, [i64]),
+    "omx_mlp_backward_f32": (i32, [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, vp]),
+    "omx_mlp_adam_f32": (i32, [i64, vp, vp, vp, vp, flt, flt, flt, flt, i32, vp]),
+    "omx_sched_features_f32": (i32, [i64, vp, i64, vp, vp, i64, i64, i64, i64, i64, i32, vp, vp]),
     "omx_solver_create": (i32, [i64, i64, i32, i32, C.POINTER(vp)]),
     "omx_solver_destroy": (i32, [vp]),
     "omx_lu_factor": (i32, [vp, vp, i64, vp, vp]),
+    "omx_solver_set_policy": (i32, [vp, vp]),
+    "omx_solver_last_schedule": (i32, [vp, vp, vp]),
     "omx_ir_solve": (i32, [vp, vp, i64, u64, vp, vp, vp, i32, i32, i32, dbl, dbl, i32, vp, vp]),
+    "omx_ir_solve_system": (i32, [vp, vp, i64, vp, vp, vp, i32, i32, i32, dbl, dbl, i32, vp, vp]),
     "omx_harness": (dbl, [dbl, dbl, dbl, dbl, i64]),
 }
+
+
+
+class System(C.Structure):
+    """struct omx_system (include/omxp.h)."""
+    _fields_ = [("kind", C.c_int), ("seed", C.c_uint64), ("diag", vp), ("norm_inf", C.c_double),
+                ("rank", C.c_int64), ("sigma", vp), ("P", vp), ("Q", vp)]
+
+
+SYSTEM_LCG = 0
+SYSTEM_LOWRANK = 1
 
 _lib = None
 

@@ -37,6 +37,8 @@ LIB_SOURCES = [
     "omx_pack.hip",
     "omx_gemm.hip",
     "omx_dense.hip",
+    "omx_randsvd.hip",
+    "omx_mlp.hip",
     "omx_solver.cpp",
 ]
 DRIVER_SOURCES = ["omxp_main.cpp"]

@@ -18,6 +18,8 @@
 // The MFMA's A operand is the Y fragment and its B operand the X fragment,
 // so in the 32 x 32 result lane l holds row i = l%32 of C and 16 columns:
 // each store of a register writes 32 consecutive rows of one column.
+#include <type_traits>
+
 #include "omx_common.hpp"
 #include "omx_layout.hpp"
 
@@ -223,9 +225,16 @@ k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char 
     half8 xa[4], ya[4], xb[4], yb[4];
     frags(xa, ya, 0, 0);
     // The LDS-DMA loads make the compiler wait for all LDS reads before an
-    // MFMA that uses any of them, so the next fragments are read in between
-    // the current MFMAs (2 MFMAs, 1 read): they have landed when needed.
-    for (int st = 0; st < nks; ++st) {
+    // MFMA that uses any of them, so fragments are read one k-step ahead, in
+    // between the current MFMAs: they have landed when needed.  One stage: the first k-step's 16 MFMAs with the second k-step's
+    // fragment reads between them; then the wait for the next stage and the
+    // stage barrier; then the second k-step's 16 MFMAs with the refill of the
+    // freed ring slot (8 LDS-DMA loads) and the next stage's first fragment
+    // reads between them, so that neither the load issue nor the LDS reads
+    // leave the matrix pipeline idle.  kSteady: a later stage exists and the
+    // ring slot is refilled (every stage but the last kNbuf - 1).
+    auto stage = [&](int st, auto steady_tag) {
+        constexpr bool kSteady = decltype(steady_tag)::value;
         frags(xb, yb, st, 1);
 #pragma unroll
         for (int mi = 0; mi < 4; ++mi)
@@ -238,28 +247,40 @@ k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char 
             __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 DS read
         }
         __builtin_amdgcn_sched_barrier(0);
-        const bool more = st + 1 < nks;
-        if (more) {
+        const bool more = kSteady || st + 1 < nks;
+        if (kSteady) {
+            wait_stages(kNbuf - 3);
+            __syncthreads();
+        } else if (more) {
             wait_stages(min(kNbuf - 3, nks - st - 2));
             __syncthreads();
-            if (st + kNbuf - 1 < nks) issue(st + kNbuf - 1);
         }
         __builtin_amdgcn_sched_barrier(0);
-        if (more) frags(xa, ya, st + 1, 0);
+        if (kSteady) {
+            issue(st + kNbuf - 1);
+            frags(xa, ya, st + 1, 0);
+        } else if (more) {
+            if (st + kNbuf - 1 < nks) issue(st + kNbuf - 1);
+            frags(xa, ya, st + 1, 0);
+        }
 #pragma unroll
         for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
             for (int ni = 0; ni < 4; ++ni)
                 acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(yb[ni], xb[mi], acc[mi][ni], 0, 0, 0);
-        if (more) {
+        if (kSteady) {
 #pragma unroll
             for (int g = 0; g < 8; ++g) {
-                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // 2 MFMA
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 DS read
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 VMEM load
             }
         }
         __builtin_amdgcn_sched_barrier(0);
-    }
+    };
+    int st = 0;
+    for (; st + kNbuf - 1 < nks; ++st) stage(st, std::true_type{});
+    for (; st < nks; ++st) stage(st, std::false_type{});
     epilogue<kBetaZero>(acc, M, N, alpha, beta, C, ldc, tm, tn, wm, wn, lane);
 }
 
@@ -324,8 +345,12 @@ k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *
     __syncthreads();
     intx8 xa[4], ya[4], xb[4], yb[4];
     frags(xa, ya, 0);
-    // E8M0 scales: value = fp8 * 2^(s - 127) with s = 127 - e
-    auto body = [&](intx8(&xc)[4], intx8(&yc)[4], intx8(&xn)[4], intx8(&yn)[4], int cs) {
+    // E8M0 scales: value = fp8 * 2^(s - 127) with s = 127 - e.  One stage:
+    // 8 MFMAs; the wait for the next stage and the stage barrier; 8 MFMAs
+    // with the refill of the freed ring slot and the next stage's fragment
+    // reads between them (kSteady: as for FP16).
+    auto body = [&](intx8(&xc)[4], intx8(&yc)[4], intx8(&xn)[4], intx8(&yn)[4], int cs, auto steady_tag) {
+        constexpr bool kSteady = decltype(steady_tag)::value;
         const int kt = (cs * 64) / OMX_SCALE_TILE;
         const int sx = 127 - xe[kt];
         const int sy = 127 - ye[kt];
@@ -337,33 +362,47 @@ k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *
                 acc[mi][ni] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(yc[ni], xc[mi], acc[mi][ni], 0, 0, 0,
                                                                               sy, 0, sx);
         __builtin_amdgcn_sched_barrier(0);
-        const bool more = cs + 1 < nks;
-        if (more) {
+        const bool more = kSteady || cs + 1 < nks;
+        if (kSteady) {
+            wait_stages(kNbuf - 3);
+            __syncthreads();
+        } else if (more) {
             wait_stages(min(kNbuf - 3, nks - cs - 2));
             __syncthreads();
-            if (cs + kNbuf - 1 < nks) issue(cs + kNbuf - 1);
         }
         __builtin_amdgcn_sched_barrier(0);
-        if (more) frags(xn, yn, cs + 1);
+        if (kSteady) {
+            issue(cs + kNbuf - 1);
+            frags(xn, yn, cs + 1);
+        } else if (more) {
+            if (cs + kNbuf - 1 < nks) issue(cs + kNbuf - 1);
+            frags(xn, yn, cs + 1);
+        }
 #pragma unroll
         for (int mi = 2; mi < 4; ++mi)
 #pragma unroll
             for (int ni = 0; ni < 4; ++ni)
                 acc[mi][ni] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(yc[ni], xc[mi], acc[mi][ni], 0, 0, 0,
                                                                               sy, 0, sx);
-        if (more) {
+        if (kSteady) {
 #pragma unroll
             for (int g = 0; g < 8; ++g) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
                 __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // 2 DS reads
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 VMEM load
             }
         }
         __builtin_amdgcn_sched_barrier(0);
     };
     // nks is a multiple of 4 (k % 256 == 0): stages come in pairs
-    for (int st = 0; st < nks; st += 2) {
-        body(xa, ya, xb, yb, st);
-        body(xb, yb, xa, ya, st + 1);
+    int st = 0;
+    for (; st + kNbuf < nks; st += 2) {
+        body(xa, ya, xb, yb, st, std::true_type{});
+        body(xb, yb, xa, ya, st + 1, std::true_type{});
+    }
+    for (; st < nks; st += 2) {
+        body(xa, ya, xb, yb, st, std::false_type{});
+        body(xb, yb, xa, ya, st + 1, std::false_type{});
     }
     epilogue<kBetaZero>(acc, M, N, alpha, beta, C, ldc, tm, tn, wm, wn, lane);
 }

@@ -116,99 +116,232 @@ __global__ void k_zero(int64_t n, double *__restrict__ y) {
 }
 
 // ---------------------------------------------------------------------------
-// Blocked triangular solves.  The diagonal b x b block is solved by one
-// workgroup of 1024 threads in sub-blocks of 64: wave 0 solves the 64 x 64
-// triangle with lane-to-lane broadcasts, then all threads update the rows
-// of the block below (lower) or above (upper) the sub-block.  The vector
-// lives in LDS (b <= kTrsvMaxB).
+// Triangular solves with the FP32 factors, FP64 vector and arithmetic.
+//
+// The matrix is walked in blocks of kTB = 256 rows/columns.  One launch per
+// block column J (solved values x_J final in x):
+//   workgroup 0   rows of the next diagonal block D: r_D -= M(D, J) x_J, then
+//                 the diagonal block is solved in LDS (4 sub-blocks of 64,
+//                 each a lane-to-lane chain in one wave, followed by the
+//                 update of the later sub-blocks' rows), x_D written back;
+//   workgroup w   the remaining rows beyond D, 256 per workgroup:
+//                 r -= M(rows, J) x_J.
+// So the diagonal chain of block D runs concurrently with the bulk update by
+// block J, and the solve reads each FP32 factor entry exactly once (HBM-bound:
+// 2 n^2 bytes per triangle).  Summation order per row and block column: the
+// four waves each sum 64 columns in order, the partial sums are added in
+// wave order, the total is subtracted from r.
 // ---------------------------------------------------------------------------
-constexpr int kTrsvThreads = 1024;
-constexpr int kTrsvSub = 64;
-constexpr int kTrsvMaxB = 8192;
+constexpr int kTB = 256;
+constexpr int kSB = 64;
 
-__global__ void __launch_bounds__(kTrsvThreads)
-k_trsv_diag_lower(int64_t b, const float *__restrict__ A, int64_t lda,
-                  double *__restrict__ x, const double *__restrict__ acc) {
-    extern __shared__ double xs[];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    for (int64_t i = tid; i < b; i += kTrsvThreads) xs[i] = x[i] - acc[i];
-    __syncthreads();
-    for (int64_t sb = 0; sb < b; sb += kTrsvSub) {
-        if (wave == 0) {
-            const int64_t row = sb + lane;
-            float a[kTrsvSub];
+// Lane l's value of v in every lane (l uniform): two readlanes.
+__device__ __forceinline__ double bcast_lane(double v, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// sum over the block's columns of M(row, c) x_c for 256 rows from r0 (rows
+// >= rmax ignored) and ncols <= 256 columns from c0; xv (LDS) holds x of the
+// columns.  Wave w takes columns [64 w, 64 w + 64), lane l rows 4l .. 4l+3
+// (one 16-byte load per column when kVec).  Returns the row's total in
+// thread t = row - r0 (after the LDS reduction).
+template <bool kVec>
+__device__ __forceinline__ double block_rows_dot(const float *__restrict__ A, int64_t lda, int64_t r0,
+                                                 int64_t rmax, int64_t c0, int ncols, const double *xv,
+                                                 double (*part)[kTB]) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int cb = wave * kSB;
+    const int ce = min(ncols, cb + kSB);
+    const int64_t row = r0 + 4 * lane;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (kVec && row + 3 < rmax) {
+        const float *p = A + row + (c0 + cb) * lda;
+        int c = cb;
+        for (; c + 8 <= ce; c += 8) {
+            float4 v[8];
 #pragma unroll
-            for (int l = 0; l < kTrsvSub; ++l) a[l] = (lane > l) ? A[row + (sb + l) * lda] : 0.0f;
-            double v = xs[row];
+            for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const float4 *>(p + (int64_t)(c - cb + u) * lda);
 #pragma unroll
-            for (int l = 0; l < kTrsvSub; ++l) {
-                const double t = __shfl(v, l, 64);
-                if (lane > l) v -= t * (double)a[l];
+            for (int u = 0; u < 8; ++u) {
+                const double xc = xv[c + u];
+                s0 = __builtin_fma((double)v[u].x, xc, s0);
+                s1 = __builtin_fma((double)v[u].y, xc, s1);
+                s2 = __builtin_fma((double)v[u].z, xc, s2);
+                s3 = __builtin_fma((double)v[u].w, xc, s3);
             }
-            xs[row] = v;
+        }
+        for (; c < ce; ++c) {
+            const float4 v = *reinterpret_cast<const float4 *>(p + (int64_t)(c - cb) * lda);
+            const double xc = xv[c];
+            s0 = __builtin_fma((double)v.x, xc, s0);
+            s1 = __builtin_fma((double)v.y, xc, s1);
+            s2 = __builtin_fma((double)v.z, xc, s2);
+            s3 = __builtin_fma((double)v.w, xc, s3);
+        }
+    } else {
+        for (int c = cb; c < ce; ++c) {
+            const float *q = A + row + (c0 + c) * lda;
+            const double xc = xv[c];
+            if (row < rmax) s0 = __builtin_fma((double)q[0], xc, s0);
+            if (row + 1 < rmax) s1 = __builtin_fma((double)q[1], xc, s1);
+            if (row + 2 < rmax) s2 = __builtin_fma((double)q[2], xc, s2);
+            if (row + 3 < rmax) s3 = __builtin_fma((double)q[3], xc, s3);
+        }
+    }
+    part[wave][4 * lane] = s0;
+    part[wave][4 * lane + 1] = s1;
+    part[wave][4 * lane + 2] = s2;
+    part[wave][4 * lane + 3] = s3;
+    __syncthreads();
+    const double t = ((part[0][tid] + part[1][tid]) + part[2][tid]) + part[3][tid];
+    __syncthreads();
+    return t;
+}
+
+// x_D <- L_DD^-1 r_D for the unit-lower diagonal block at d0 of order dim
+// (<= 256); r_D in xs (LDS).  Sub-block sb: wave sb runs the 64-step chain
+// (lane l holds row sb*64 + l; x_c read from lane c), then the waves of
+// the later sub-blocks subtract M(rows, sub-block) x_sb.
+__device__ __forceinline__ void diag_solve_lower(const float *__restrict__ A, int64_t lda, int64_t d0, int dim,
+                                                 double *xs) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int sb = 0; sb * kSB < dim; ++sb) {
+        const int c0 = sb * kSB;
+        if (wave == sb) {
+            const int i = c0 + lane;
+            float a[kSB];
+            const float *p = A + d0 + min(i, dim - 1);
+#pragma unroll
+            for (int c = 0; c < kSB; ++c)  // row and column clamped into the block
+                a[c] = (lane > c && i < dim) ? p[(d0 + min(c0 + c, dim - 1)) * lda] : 0.0f;
+            double v = xs[i];
+            // a[c] is zero for lanes <= c, so the update needs no lane mask
+#pragma unroll
+            for (int c = 0; c < kSB; ++c) v = __builtin_fma(-bcast_lane(v, c), (double)a[c], v);
+            xs[i] = v;
         }
         __syncthreads();
-        for (int64_t i = sb + kTrsvSub + tid; i < b; i += kTrsvThreads) {
-            double s = 0.0;
+        if (wave > sb) {
+            const int i = wave * kSB + lane;
+            if (i < dim) {
+                const float *p = A + d0 + i + (d0 + c0) * lda;
+                double s = 0.0;
+                const int ce = min(kSB, dim - c0);
 #pragma unroll 16
-            for (int l = 0; l < kTrsvSub; ++l) s += (double)A[i + (sb + l) * lda] * xs[sb + l];
+                for (int c = 0; c < ce; ++c) s = __builtin_fma((double)p[(int64_t)c * lda], xs[c0 + c], s);
+                xs[i] -= s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// x_D <- U_DD^-1 r_D for the upper non-unit diagonal block (sub-blocks from
+// the last to the first).  The quotient r / u is formed as q0 = r * y,
+// q = q0 + (r - u q0) y with y = 1/u: the correctly rounded quotient
+// (Markstein's correction with a correctly rounded reciprocal).
+__device__ __forceinline__ void diag_solve_upper(const float *__restrict__ A, int64_t lda, int64_t d0, int dim,
+                                                 double *xs) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nsb = (dim + kSB - 1) / kSB;
+    for (int sb = nsb - 1; sb >= 0; --sb) {
+        const int c0 = sb * kSB;
+        if (wave == sb) {
+            const int i = c0 + lane;
+            float a[kSB];  // strictly upper part of the lane's row
+            const float *p = A + d0 + min(i, dim - 1);
+#pragma unroll
+            for (int c = 0; c < kSB; ++c)  // row and column clamped into the block
+                a[c] = (lane < c && c0 + c < dim) ? p[(d0 + min(c0 + c, dim - 1)) * lda] : 0.0f;
+            const double u = (i < dim) ? (double)A[d0 + i + (d0 + i) * lda] : 1.0;
+            const double y = 1.0 / u;
+            auto quot = [&](double r) {
+                const double q0 = r * y;
+                return __builtin_fma(__builtin_fma(-u, q0, r), y, q0);
+            };
+            // Step c: every lane forms its quotient, lane c's is x_c; rows
+            // above subtract x_c u(i, c).  Lane c's own v is untouched from
+            // then on, so its final quotient below equals x_c.
+            double v = (i < dim) ? xs[i] : 0.0;
+#pragma unroll
+            for (int c = kSB - 1; c >= 0; --c) v = __builtin_fma(-bcast_lane(quot(v), c), (double)a[c], v);
+            if (i < dim) xs[i] = quot(v);
+        }
+        __syncthreads();
+        if (wave < sb) {
+            const int i = wave * kSB + lane;
+            const float *p = A + d0 + i + (d0 + c0) * lda;
+            double s = 0.0;
+            const int ce = min(kSB, dim - c0);
+#pragma unroll 16
+            for (int c = 0; c < ce; ++c) s = __builtin_fma((double)p[(int64_t)c * lda], xs[c0 + c], s);
             xs[i] -= s;
         }
         __syncthreads();
     }
-    for (int64_t i = tid; i < b; i += kTrsvThreads) x[i] = xs[i];
 }
 
-__global__ void __launch_bounds__(kTrsvThreads)
-k_trsv_diag_upper(int64_t b, const float *__restrict__ A, int64_t lda,
-                  double *__restrict__ x, const double *__restrict__ acc) {
-    extern __shared__ double xs[];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    for (int64_t i = tid; i < b; i += kTrsvThreads) xs[i] = x[i] - acc[i];
-    __syncthreads();
-    for (int64_t sb = b - kTrsvSub; sb >= 0; sb -= kTrsvSub) {
-        if (wave == 0) {
-            const int64_t row = sb + lane;
-            float a[kTrsvSub];
-#pragma unroll
-            for (int l = 0; l < kTrsvSub; ++l) a[l] = (lane <= l) ? A[row + (sb + l) * lda] : 0.0f;
-            double v = xs[row];
-#pragma unroll
-            for (int l = kTrsvSub - 1; l >= 0; --l) {
-                if (lane == l) v = v / (double)a[l];
-                const double t = __shfl(v, l, 64);
-                if (lane < l) v -= t * (double)a[l];
-            }
-            xs[row] = v;
-        }
-        __syncthreads();
-        for (int64_t i = tid; i < sb; i += kTrsvThreads) {
-            double s = 0.0;
-#pragma unroll 16
-            for (int l = 0; l < kTrsvSub; ++l) s += (double)A[i + (sb + l) * lda] * xs[sb + l];
-            xs[i] -= s;
-        }
-        __syncthreads();
-    }
-    for (int64_t i = tid; i < b; i += kTrsvThreads) x[i] = xs[i];
-}
-
-// acc_i += sum_l A(i, l) * x_l for rows i < m, l < b (FP32 matrix, FP64
-// vector and sums); one thread per row, columns in order.
+// One block column of the forward solve.  j0 = first column of the solved
+// block J (j0 < 0: nothing solved yet, only the first diagonal block).
+template <bool kVec>
 __global__ void __launch_bounds__(256)
-k_trsv_gemv(int64_t m, int64_t b, const float *__restrict__ A, int64_t lda,
-            const double *__restrict__ xv, double *__restrict__ acc) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= m) return;
-    double s = 0.0;
-    const float *p = A + i;
-#pragma unroll 8
-    for (int64_t l = 0; l < b; ++l) s += (double)p[l * lda] * xv[l];
-    acc[i] += s;
+k_trsv_lower_step(int64_t n, int64_t j0, const float *__restrict__ A, int64_t lda, double *__restrict__ x) {
+    __shared__ double xv[kTB];
+    __shared__ double xs[kTB];
+    __shared__ double part[4][kTB];
+    const int tid = threadIdx.x;
+    const int64_t jb = j0 < 0 ? 0 : j0;
+    const int ncols = j0 < 0 ? 0 : (int)min((int64_t)kTB, n - j0);
+    const int64_t d0 = j0 < 0 ? 0 : j0 + kTB;
+    if (ncols > 0) xv[tid] = tid < ncols ? x[jb + tid] : 0.0;
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        const int dim = (int)min((int64_t)kTB, n - d0);
+        double r = (tid < dim) ? x[d0 + tid] : 0.0;
+        if (ncols > 0) r -= block_rows_dot<kVec>(A, lda, d0, d0 + dim, jb, ncols, xv, part);
+        xs[tid] = r;
+        __syncthreads();
+        diag_solve_lower(A, lda, d0, dim, xs);
+        if (tid < dim) x[d0 + tid] = xs[tid];
+        return;
+    }
+    const int64_t r0 = d0 + (int64_t)blockIdx.x * kTB;
+    const double t = block_rows_dot<kVec>(A, lda, r0, n, jb, ncols, xv, part);
+    if (r0 + tid < n) x[r0 + tid] -= t;
+}
+
+// One block column of the backward solve: J = [j0, j0 + ncols) solved
+// (j0 >= n: nothing solved yet, only the last diagonal block, which may be
+// shorter than kTB); D = the block before J; rows [0, d0) updated.
+template <bool kVec>
+__global__ void __launch_bounds__(256)
+k_trsv_upper_step(int64_t n, int64_t j0, const float *__restrict__ A, int64_t lda, double *__restrict__ x) {
+    __shared__ double xv[kTB];
+    __shared__ double xs[kTB];
+    __shared__ double part[4][kTB];
+    const int tid = threadIdx.x;
+    const int64_t last0 = ((n - 1) / kTB) * kTB;  // first row of the last block
+    const bool none = j0 >= n;
+    const int ncols = none ? 0 : (int)min((int64_t)kTB, n - j0);
+    const int64_t d0 = none ? last0 : j0 - kTB;
+    const int dim = (int)min((int64_t)kTB, n - d0);
+    if (ncols > 0) xv[tid] = tid < ncols ? x[j0 + tid] : 0.0;
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        double r = (tid < dim) ? x[d0 + tid] : 0.0;
+        if (ncols > 0) r -= block_rows_dot<kVec>(A, lda, d0, d0 + dim, j0, ncols, xv, part);
+        xs[tid] = r;
+        __syncthreads();
+        diag_solve_upper(A, lda, d0, dim, xs);
+        if (tid < dim) x[d0 + tid] = xs[tid];
+        return;
+    }
+    const int64_t r0 = (int64_t)(blockIdx.x - 1) * kTB;
+    const double t = block_rows_dot<kVec>(A, lda, r0, d0, j0, ncols, xv, part);
+    if (r0 + tid < d0) x[r0 + tid] -= t;
 }
 
 int grid_for(int64_t n, int threads) {
@@ -279,57 +412,60 @@ int omx_axpy_f64(int64_t n, double alpha, const double *x, double *y, void *stre
     return OMX_OK;
 }
 
-size_t omx_trsv_workspace_bytes(int64_t n) { return n > 0 ? (size_t)n * sizeof(double) : 0; }
+size_t omx_trsv_workspace_bytes(int64_t n) {
+    (void)n;
+    return 0;
+}
 
-static int trsv_args(int64_t n, int64_t b, const float *A, int64_t lda, double *x, void *work,
-                     size_t work_bytes) {
-    if (n <= 0 || b <= 0 || n % b || b % kTrsvSub || b > kTrsvMaxB || lda < n || !A || !x)
-        return OMX_ERR_ARG;
-    if (!work || work_bytes < omx_trsv_workspace_bytes(n)) return OMX_ERR_WORKSPACE;
+static int trsv_args(int64_t n, int64_t b, const float *A, int64_t lda, double *x) {
+    if (n <= 0 || b <= 0 || n % b || lda < n || !A || !x) return OMX_ERR_ARG;
     return OMX_OK;
+}
+
+static bool trsv_vec(const float *A, int64_t lda) {
+    return (reinterpret_cast<uintptr_t>(A) & 15) == 0 && (lda & 3) == 0;
 }
 
 int omx_trsv_lower_unit_f32f64(int64_t n, int64_t b, const float *A, int64_t lda, double *x,
                                void *work, size_t work_bytes, void *stream) {
-    int rc = trsv_args(n, b, A, lda, x, work, work_bytes);
+    (void)work;
+    (void)work_bytes;
+    int rc = trsv_args(n, b, A, lda, x);
     if (rc) return rc;
     hipStream_t st = omx_stream(stream);
-    double *acc = static_cast<double *>(work);
-    hipLaunchKernelGGL(k_zero, dim3(grid_for(n, 256)), dim3(256), 0, st, n, acc);
-    OMX_LAUNCH_CHECK();
-    const size_t lds = (size_t)b * sizeof(double);
-    for (int64_t j0 = 0; j0 < n; j0 += b) {
-        hipLaunchKernelGGL(k_trsv_diag_lower, dim3(1), dim3(kTrsvThreads), lds, st, b,
-                           A + j0 + j0 * lda, lda, x + j0, acc + j0);
+    const bool vec = trsv_vec(A, lda);
+    for (int64_t j0 = -kTB; j0 + kTB < n; j0 += kTB) {
+        const int64_t d0 = j0 + kTB;
+        const int64_t beyond = n - d0 - kTB;
+        const unsigned grid = 1u + (unsigned)(beyond > 0 && j0 >= 0 ? omx_cdiv(beyond, kTB) : 0);
+        if (vec)
+            hipLaunchKernelGGL(k_trsv_lower_step<true>, dim3(grid), dim3(256), 0, st, n, j0, A, lda, x);
+        else
+            hipLaunchKernelGGL(k_trsv_lower_step<false>, dim3(grid), dim3(256), 0, st, n, j0, A, lda, x);
         OMX_LAUNCH_CHECK();
-        const int64_t rest = n - j0 - b;
-        if (rest > 0) {
-            hipLaunchKernelGGL(k_trsv_gemv, dim3((unsigned)omx_cdiv(rest, 256)), dim3(256), 0, st,
-                               rest, b, A + (j0 + b) + j0 * lda, lda, x + j0, acc + j0 + b);
-            OMX_LAUNCH_CHECK();
-        }
     }
     return OMX_OK;
 }
 
 int omx_trsv_upper_f32f64(int64_t n, int64_t b, const float *A, int64_t lda, double *x,
                           void *work, size_t work_bytes, void *stream) {
-    int rc = trsv_args(n, b, A, lda, x, work, work_bytes);
+    (void)work;
+    (void)work_bytes;
+    int rc = trsv_args(n, b, A, lda, x);
     if (rc) return rc;
     hipStream_t st = omx_stream(stream);
-    double *acc = static_cast<double *>(work);
-    hipLaunchKernelGGL(k_zero, dim3(grid_for(n, 256)), dim3(256), 0, st, n, acc);
-    OMX_LAUNCH_CHECK();
-    const size_t lds = (size_t)b * sizeof(double);
-    for (int64_t j0 = n - b; j0 >= 0; j0 -= b) {
-        hipLaunchKernelGGL(k_trsv_diag_upper, dim3(1), dim3(kTrsvThreads), lds, st, b,
-                           A + j0 + j0 * lda, lda, x + j0, acc + j0);
+    const bool vec = trsv_vec(A, lda);
+    const int64_t last0 = ((n - 1) / kTB) * kTB;
+    // j0 = n: the last block alone; then J = each block from the last down
+    for (int64_t j0 = n; j0 > 0; j0 = (j0 == n) ? last0 : j0 - kTB) {
+        const int64_t d0 = (j0 == n) ? last0 : j0 - kTB;
+        if (d0 < 0) break;
+        const unsigned grid = 1u + (unsigned)(j0 < n ? omx_cdiv(d0, kTB) : 0);
+        if (vec)
+            hipLaunchKernelGGL(k_trsv_upper_step<true>, dim3(grid), dim3(256), 0, st, n, j0, A, lda, x);
+        else
+            hipLaunchKernelGGL(k_trsv_upper_step<false>, dim3(grid), dim3(256), 0, st, n, j0, A, lda, x);
         OMX_LAUNCH_CHECK();
-        if (j0 > 0) {
-            hipLaunchKernelGGL(k_trsv_gemv, dim3((unsigned)omx_cdiv(j0, 256)), dim3(256), 0, st,
-                               j0, b, A + j0 * lda, lda, x + j0, acc);
-            OMX_LAUNCH_CHECK();
-        }
     }
     return OMX_OK;
 }

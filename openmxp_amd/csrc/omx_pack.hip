@@ -130,9 +130,12 @@ __device__ __forceinline__ int pow2_exp_448(float maxabs) {
     return e > 100 ? 100 : (e < -100 ? -100 : e);
 }
 
-__global__ void k_tile_exps(int64_t n, const float *__restrict__ tmax, int32_t *__restrict__ exps) {
+__global__ void k_tile_exps(int64_t n, const float *__restrict__ tmax, int headroom, int32_t *__restrict__ exps) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) exps[i] = pow2_exp_448(tmax[i]);
+    if (i < n) {
+        const int e = pow2_exp_448(tmax[i]);
+        exps[i] = (tmax[i] > 0.0f) ? (e - headroom < -100 ? -100 : e - headroom) : 0;
+    }
 }
 
 // FP8 pack: one workgroup per (row strip, 64-column half of a 128 stage).
@@ -198,21 +201,41 @@ int omx_pack_f16(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t 
     return OMX_OK;
 }
 
-int omx_pack_f8(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t ls, uint8_t *P,
-                int32_t *exps, float *tile_max, void *stream) {
+int omx_pack_f8_ex(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t ls, int headroom, int have_max,
+                   uint8_t *P, int32_t *exps, float *tile_max, void *stream) {
     if (rows <= 0 || k <= 0 || k % OMX_SCALE_TILE || !src || !P || !exps || !tile_max) return OMX_ERR_ARG;
+    if (headroom < 0 || headroom > 3) return OMX_ERR_ARG;
     if (!((rs == 1 && ls >= rows) || (ls == 1 && rs >= k))) return OMX_ERR_ARG;
     if (ls == 1 && (rs % 4 || (reinterpret_cast<uintptr_t>(src) & 15))) return OMX_ERR_ARG;
     hipStream_t st = omx_stream(stream);
     const int64_t strips = omx_strips(rows), KT = omx_cdiv(k, OMX_SCALE_TILE);
-    hipLaunchKernelGGL(k_tile_maxabs, dim3((unsigned)KT, (unsigned)strips), dim3(kPackThreads), 0, st,
-                       rows, k, src, rs, ls, tile_max);
-    OMX_LAUNCH_CHECK();
+    if (!have_max) {
+        hipLaunchKernelGGL(k_tile_maxabs, dim3((unsigned)KT, (unsigned)strips), dim3(kPackThreads), 0, st,
+                           rows, k, src, rs, ls, tile_max);
+        OMX_LAUNCH_CHECK();
+    }
     const int64_t nt = strips * KT;
-    hipLaunchKernelGGL(k_tile_exps, dim3((unsigned)omx_cdiv(nt, 256)), dim3(256), 0, st, nt, tile_max, exps);
+    hipLaunchKernelGGL(k_tile_exps, dim3((unsigned)omx_cdiv(nt, 256)), dim3(256), 0, st, nt, tile_max, headroom,
+                       exps);
     OMX_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_pack_f8, dim3((unsigned)(k / kTileK), (unsigned)strips), dim3(kPackThreads), 0,
                        st, rows, k, src, rs, ls, exps, P);
+    OMX_LAUNCH_CHECK();
+    return OMX_OK;
+}
+
+int omx_pack_f8(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t ls, uint8_t *P,
+                int32_t *exps, float *tile_max, void *stream) {
+    return omx_pack_f8_ex(rows, k, src, rs, ls, 0, 0, P, exps, tile_max, stream);
+}
+
+int omx_panel_tile_maxabs(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t ls, float *tile_max,
+                          void *stream) {
+    if (rows <= 0 || k <= 0 || k % OMX_SCALE_TILE || !src || !tile_max) return OMX_ERR_ARG;
+    if (!((rs == 1 && ls >= rows) || (ls == 1 && rs >= k))) return OMX_ERR_ARG;
+    const int64_t strips = omx_strips(rows), KT = omx_cdiv(k, OMX_SCALE_TILE);
+    hipLaunchKernelGGL(k_tile_maxabs, dim3((unsigned)KT, (unsigned)strips), dim3(kPackThreads), 0,
+                       omx_stream(stream), rows, k, src, rs, ls, tile_max);
     OMX_LAUNCH_CHECK();
     return OMX_OK;
 }

@@ -60,6 +60,13 @@ struct omx_solver {
     DevBuf Wup, Wlo, Wup_h, Wlo_h;      // -alt inverses (FP32, packed FP16)
     DevBuf cmaxL, cmaxR;                // chunk max-abs of the FP16 packs
     DevBuf info;
+    // learned schedule: MLP parameters, per-step features, forward scratch
+    DevBuf pol, feat, mlp_h, zdev;
+    int has_policy = 0;
+    hipEvent_t ev_dec = nullptr;
+    float *zhost = nullptr;            // pinned: the step's 3 outputs
+    std::vector<int> last_sched;       // per step: 0 FP16, 1 + headroom FP8
+    std::vector<float> last_feat;      // per step: 16 features (FP8 solvers)
     // refinement
     DevBuf r, w, gemv_ws, trsv_ws, nrm, V, hv, dots;
     int restart_cap = 0;
@@ -204,7 +211,14 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
     if (s->fp8) {
         const size_t ne = omx_panel_exps_count(rows, b);
         if ((rc = dalloc(s->L8max, ne * 4)) || (rc = dalloc(s->R8max, ne * 4))) return fail(rc);
+        if ((rc = dalloc(s->pol, OMX_MLP_NPARAMS * 4)) || (rc = dalloc(s->feat, (size_t)nb * OMX_MLP_NFEATURES * 4)) ||
+            (rc = dalloc(s->mlp_h, 2 * 64 * 4)) || (rc = dalloc(s->zdev, 16)))
+            return fail(rc);
+        if (hipEventCreateWithFlags(&s->ev_dec, hipEventDisableTiming) != hipSuccess) return fail(OMX_ERR_HIP);
+        if (hipHostMalloc(reinterpret_cast<void **>(&s->zhost), 16) != hipSuccess) return fail(OMX_ERR_HIP);
     }
+    s->last_sched.assign(nb, 0);
+    s->last_feat.assign((size_t)nb * OMX_MLP_NFEATURES, 0.0f);
     if ((rc = dalloc(s->Tp, pb16))) return fail(rc);
     if ((rc = dalloc(s->Wup, (size_t)b * b * 4)) || (rc = dalloc(s->Wlo, (size_t)b * b * 4)) ||
         (rc = dalloc(s->Wup_h, omx_panel_bytes(b, b, 0))) || (rc = dalloc(s->Wlo_h, omx_panel_bytes(b, b, 0))))
@@ -214,7 +228,7 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
     if ((rc = dalloc(s->info, 16))) return fail(rc);
     // refinement vectors
     if ((rc = dalloc(s->r, n * 8)) || (rc = dalloc(s->w, n * 8)) ||
-        (rc = dalloc(s->gemv_ws, std::max<size_t>(omx_otf_gemv_workspace_bytes(n), 8))) ||
+        (rc = dalloc(s->gemv_ws, std::max(std::max<size_t>(omx_otf_gemv_workspace_bytes(n), 8), omx_lowrank_gemv_workspace_bytes(16)))) ||
         (rc = dalloc(s->trsv_ws, omx_trsv_workspace_bytes(n))) || (rc = dalloc(s->nrm, 64)))
         return fail(rc);
     *out = s;
@@ -231,8 +245,11 @@ int omx_solver_destroy(omx_solver *s) {
     DevBuf *bufs[] = {&s->Lp[0], &s->Lp[1], &s->Rp[0], &s->Rp[1], &s->Tp, &s->L8[0], &s->L8[1], &s->R8[0],
                       &s->R8[1], &s->Le[0], &s->Le[1], &s->Re[0], &s->Re[1], &s->L8max, &s->R8max,
                       &s->Wup, &s->Wlo, &s->Wup_h, &s->Wlo_h, &s->cmaxL, &s->cmaxR, &s->info, &s->r,
-                      &s->w, &s->gemv_ws, &s->trsv_ws, &s->nrm, &s->V, &s->hv, &s->dots};
+                      &s->w, &s->gemv_ws, &s->trsv_ws, &s->nrm, &s->V, &s->hv, &s->dots,
+                      &s->pol, &s->feat, &s->mlp_h, &s->zdev};
     for (DevBuf *d : bufs) dfree(*d);
+    if (s->ev_dec) (void)hipEventDestroy(s->ev_dec);
+    if (s->zhost) (void)hipHostFree(s->zhost);
     if (s->crit) (void)hipStreamDestroy(s->crit);
     if (s->bulk) (void)hipStreamDestroy(s->bulk);
     delete s;
@@ -253,14 +270,14 @@ static int schur(omx_solver *s, int use8, int cur, int64_t m, int64_t nn, int64_
 }
 
 // Pack the Schur panels of step k from the finished FP32 column / row panels.
-static int pack_panels(omx_solver *s, int nxt, int use8, int64_t rest, const float *Acol, const float *Arow,
-                       int64_t lda, hipStream_t st) {
+static int pack_panels(omx_solver *s, int nxt, int use8, int headroom, int have_max, int64_t rest,
+                       const float *Acol, const float *Arow, int64_t lda, hipStream_t st) {
     const int64_t b = s->b;
     if (use8) {
-        RC(omx_pack_f8(rest, b, Acol, 1, lda, static_cast<uint8_t *>(s->L8[nxt].p),
-                       static_cast<int32_t *>(s->Le[nxt].p), static_cast<float *>(s->L8max.p), st));
-        RC(omx_pack_f8(rest, b, Arow, lda, 1, static_cast<uint8_t *>(s->R8[nxt].p),
-                       static_cast<int32_t *>(s->Re[nxt].p), static_cast<float *>(s->R8max.p), st));
+        RC(omx_pack_f8_ex(rest, b, Acol, 1, lda, headroom, have_max, static_cast<uint8_t *>(s->L8[nxt].p),
+                          static_cast<int32_t *>(s->Le[nxt].p), static_cast<float *>(s->L8max.p), st));
+        RC(omx_pack_f8_ex(rest, b, Arow, lda, 1, headroom, have_max, static_cast<uint8_t *>(s->R8[nxt].p),
+                          static_cast<int32_t *>(s->Re[nxt].p), static_cast<float *>(s->R8max.p), st));
         return OMX_OK;
     }
     RC(omx_pack_f16(rest, b, Acol, 1, lda, OMX_MASK_NONE, static_cast<uint16_t *>(s->Lp[nxt].p),
@@ -270,13 +287,77 @@ static int pack_panels(omx_solver *s, int nxt, int use8, int64_t rest, const flo
     return OMX_OK;
 }
 
+// Statistics of step k's finished panels (FP8 solvers): tile maxima of the
+// column and row panels and the 16 scheduler features; with a policy, the
+// MLP's outputs are copied to pinned host memory behind s->ev_dec.
+static int step_features(omx_solver *s, int64_t k, int64_t rest, const float *Akk, const float *Acol,
+                         const float *Arow, int64_t lda, int prev, bool infer, hipStream_t cs) {
+    const int64_t b = s->b;
+    float *lmax = static_cast<float *>(s->L8max.p), *rmax = static_cast<float *>(s->R8max.p);
+    RC(omx_panel_tile_maxabs(rest, b, Acol, 1, lda, lmax, cs));
+    RC(omx_panel_tile_maxabs(rest, b, Arow, lda, 1, rmax, cs));
+    const int64_t nt = (int64_t)omx_panel_exps_count(rest, b);
+    float *f = static_cast<float *>(s->feat.p) + k * OMX_MLP_NFEATURES;
+    RC(omx_sched_features_f32(nt, lmax, nt, rmax, Akk, lda, b, k, s->nb, s->n, prev, f, cs));
+    if (infer) {
+        float *h = static_cast<float *>(s->mlp_h.p);
+        float *z = static_cast<float *>(s->zdev.p);
+        RC(omx_mlp_forward_f32(1, static_cast<const float *>(s->pol.p), f, h, h + 64, z, cs));
+        OMX_HIP_CHECK(hipMemcpyAsync(s->zhost, z, 3 * sizeof(float), hipMemcpyDeviceToHost, cs));
+        OMX_HIP_CHECK(hipEventRecord(s->ev_dec, cs));
+    }
+    return OMX_OK;
+}
+
+// The policy's choice for the step whose outputs step_features() queued:
+// 0 FP16, 1 + headroom for FP8.
+static int policy_decision(omx_solver *s, int *choice) {
+    OMX_HIP_CHECK(hipEventSynchronize(s->ev_dec));
+    const float *z = s->zhost;
+    if (!(z[1] > z[0])) {
+        *choice = 0;
+        return OMX_OK;
+    }
+    float h = std::nearbyint(z[2]);
+    if (!(h >= 0.0f)) h = 0.0f;
+    if (h > 3.0f) h = 3.0f;
+    *choice = 1 + (int)h;
+    return OMX_OK;
+}
+
+int omx_solver_set_policy(omx_solver *s, const float *params) {
+    if (!s) return OMX_ERR_ARG;
+    if (!params) {
+        s->has_policy = 0;
+        return OMX_OK;
+    }
+    if (!s->fp8) return OMX_ERR_ARG;  // the policy needs the FP8 buffers
+    OMX_HIP_CHECK(hipMemcpy(s->pol.p, params, OMX_MLP_NPARAMS * sizeof(float), hipMemcpyHostToDevice));
+    s->has_policy = 1;
+    return OMX_OK;
+}
+
+int omx_solver_last_schedule(omx_solver *s, int *sched, float *features) {
+    if (!s) return OMX_ERR_ARG;
+    if (sched) std::memcpy(sched, s->last_sched.data(), s->last_sched.size() * sizeof(int));
+    if (features) {
+        if (!s->fp8) return OMX_ERR_ARG;
+        OMX_HIP_CHECK(hipMemcpy(features, s->feat.p, s->last_feat.size() * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return OMX_OK;
+}
+
 int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double *timings) {
     if (!s || !A || lda < s->n || lda % 4) return OMX_ERR_ARG;
     const int64_t n = s->n, b = s->b, nb = s->nb;
     hipStream_t cs = s->crit, bs = s->bulk;
-    if (sched && !s->fp8)
+    if (sched)
         for (int64_t k = 0; k < nb; ++k)
-            if (sched[k]) return OMX_ERR_ARG;  // FP8 buffers were not allocated
+            if (sched[k] < 0 || sched[k] > 4 || (sched[k] && !s->fp8)) return OMX_ERR_ARG;
+    // sched given: fixed choices; else the policy if one is set; else FP16
+    const bool learned = !sched && s->has_policy;
+    const bool stats = s->fp8 != 0;
+    auto fixed_choice = [&](int64_t k) { return sched ? sched[k] : 0; };
     int *info = static_cast<int *>(s->info.p);
     uint16_t *Tp = static_cast<uint16_t *>(s->Tp.p);
     float *Wup = static_cast<float *>(s->Wup.p), *Wlo = static_cast<float *>(s->Wlo.p);
@@ -289,14 +370,19 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
 
     // step 0: getrf(A00) and the two FP32 TRSM panels
     // (panel_trf_gdirect.hpp:699-737; panel_trf.hpp:188-246)
+    if (stats) OMX_HIP_CHECK(hipMemsetAsync(s->feat.p, 0, s->feat.bytes, cs));
     RC(omx_getrf_nopiv_f32(b, A, lda, info, cs));
-    int use8_prev = (sched && sched[0]) ? 1 : 0;
+    int choice = fixed_choice(0);
     if (nb > 1) {
         const int64_t rest = n - b;
         RC(omx_trsm_f32(OMX_TRSM_RUNN, rest, b, A, lda, A + b, lda, cs));
         RC(omx_trsm_f32(OMX_TRSM_LLNU, b, rest, A, lda, A + b * lda, lda, cs));
-        RC(pack_panels(s, 0, use8_prev, rest, A + b, A + b * lda, lda, cs));
+        if (stats) RC(step_features(s, 0, rest, A, A + b, A + b * lda, lda, 0, learned, cs));
+        if (learned) RC(policy_decision(s, &choice));
+        RC(pack_panels(s, 0, choice > 0, choice > 0 ? choice - 1 : 0, stats, rest, A + b, A + b * lda, lda, cs));
     }
+    s->last_sched[0] = choice;
+    int use8_prev = choice > 0 ? 1 : 0;
     OMX_HIP_CHECK(hipEventRecord(s->ev_panel[0], cs));
 
     for (int64_t k = 1; k < nb; ++k) {
@@ -304,7 +390,6 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
         const int64_t kb = k * b;
         const int64_t rest = n - kb - b;
         float *Akk = A + kb + kb * lda;
-        const int use8 = (sched && sched[k]) ? 1 : 0;
         // ---- critical path: diagonal block, column and row panels ----
         if (k >= 2) OMX_HIP_CHECK(hipStreamWaitEvent(cs, s->ev_bulk[k - 1], 0));
         RC(schur(s, use8_prev, cur, b, b, 0, 0, Akk, lda, cs));
@@ -336,10 +421,9 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
                 RC(omx_pack_f16(rest, b, Arow, lda, 1, OMX_MASK_NONE, Tp, nullptr, cs));
                 RC(omx_gemm_f16_packed(b, rest, b, 1.0f, Wlo_h, 0, Tp, 0, 0.0f, Arow, lda, cs));
             }
-            RC(pack_panels(s, nxt, use8, rest, Acol, Arow, lda, cs));
+            if (stats) RC(step_features(s, k, rest, Akk, Acol, Arow, lda, use8_prev, learned, cs));
             s->crit_flops += 4.0 * rest * (double)b * b;
         }
-        OMX_HIP_CHECK(hipEventRecord(s->ev_panel[k], cs));
         // ---- bulk: trailing update with the previous panels (:900-915) ----
         if (rest > 0) {
             OMX_HIP_CHECK(hipStreamWaitEvent(bs, s->ev_panel[k - 1], 0));
@@ -350,7 +434,21 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
             ++s->bulk_count;
         }
         OMX_HIP_CHECK(hipEventRecord(s->ev_bulk[k], bs));
-        use8_prev = use8;
+        // ---- this step's precision, then its Schur panels ----
+        // (with a policy the host waits here for the step's panels; the
+        // trailing update above is already queued, so the GPU stays busy)
+        choice = fixed_choice(k);
+        if (rest > 0) {
+            if (learned) RC(policy_decision(s, &choice));
+            float *Acol = A + (kb + b) + kb * lda;
+            float *Arow = A + kb + (kb + b) * lda;
+            RC(pack_panels(s, nxt, choice > 0, choice > 0 ? choice - 1 : 0, stats, rest, Acol, Arow, lda, cs));
+        } else {
+            choice = 0;
+        }
+        s->last_sched[k] = choice;
+        OMX_HIP_CHECK(hipEventRecord(s->ev_panel[k], cs));
+        use8_prev = choice > 0 ? 1 : 0;
     }
     OMX_HIP_CHECK(hipStreamSynchronize(cs));
     OMX_HIP_CHECK(hipStreamSynchronize(bs));
@@ -386,7 +484,32 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
 int omx_ir_solve(omx_solver *s, const float *LU, int64_t lda, uint64_t seed, const double *diag,
                  const double *rhs, double *x, int method, int maxit, int restart, double inner_tol,
                  double stop_harness, int x_given, double *out, double *trace) {
-    if (!s || !LU || !diag || !rhs || !x || maxit <= 0 || lda < s->n) return OMX_ERR_ARG;
+    omx_system sys{};
+    sys.kind = OMX_SYSTEM_LCG;
+    sys.seed = seed;
+    sys.diag = diag;
+    return omx_ir_solve_system(s, LU, lda, &sys, rhs, x, method, maxit, restart, inner_tol, stop_harness,
+                               x_given, out, trace);
+}
+
+// y += alpha A x for the system being refined
+static int system_gemv(omx_solver *s, const omx_system *sys, double alpha, const double *x, double *y,
+                       hipStream_t st) {
+    if (sys->kind == OMX_SYSTEM_LCG)
+        return omx_otf_gemv_f64(s->n, sys->seed, sys->diag, alpha, x, y, s->gemv_ws.p, s->gemv_ws.bytes, st);
+    return omx_lowrank_gemv_f64(s->n, sys->rank, sys->sigma, sys->P, sys->Q, alpha, x, y, s->gemv_ws.p,
+                                s->gemv_ws.bytes, st);
+}
+
+int omx_ir_solve_system(omx_solver *s, const float *LU, int64_t lda, const omx_system *sys, const double *rhs,
+                        double *x, int method, int maxit, int restart, double inner_tol, double stop_harness,
+                        int x_given, double *out, double *trace) {
+    if (!s || !LU || !sys || !sys->diag || !rhs || !x || maxit <= 0 || lda < s->n) return OMX_ERR_ARG;
+    if (sys->kind != OMX_SYSTEM_LCG &&
+        (sys->kind != OMX_SYSTEM_LOWRANK || sys->rank < 0 || sys->rank > 16 || !sys->sigma ||
+         (sys->rank > 0 && (!sys->P || !sys->Q)) || !(sys->norm_inf > 0)))
+        return OMX_ERR_ARG;
+    const double *diag = sys->diag;
     if (method == 1 && (restart <= 0 || restart > 512)) return OMX_ERR_ARG;
     const int64_t n = s->n, b = s->b;
     hipStream_t st = s->crit;
@@ -406,14 +529,14 @@ int omx_ir_solve(omx_solver *s, const float *LU, int64_t lda, uint64_t seed, con
     RC(omx_norm_inf_f64(n, rhs, nrm + 1, st));
     OMX_HIP_CHECK(hipMemcpyAsync(hn, nrm, 16, hipMemcpyDeviceToHost, st));
     OMX_HIP_CHECK(hipStreamSynchronize(st));
-    const double norma = 2.0 * hn[0], normb = hn[1];
+    const double norma = sys->kind == OMX_SYSTEM_LCG ? 2.0 * hn[0] : sys->norm_inf, normb = hn[1];
     if (!x_given) RC(omx_div_f64(n, rhs, diag, x, st));
     int inner_total = 0, result = OMX_ERR_NOCONV, it = 0;
     double normr = -1, harness = -1;
     for (it = 0; it < maxit; ++it) {
         RC(omx_copy_f64(n, rhs, r, st));
         RC(omx_norm_inf_f64(n, x, nrm, st));
-        RC(omx_otf_gemv_f64(n, seed, diag, -1.0, x, r, s->gemv_ws.p, s->gemv_ws.bytes, st));
+        RC(system_gemv(s, sys, -1.0, x, r, st));
         RC(omx_norm_inf_f64(n, r, nrm + 1, st));
         OMX_HIP_CHECK(hipMemcpyAsync(hn, nrm, 16, hipMemcpyDeviceToHost, st));
         OMX_HIP_CHECK(hipStreamSynchronize(st));
@@ -459,7 +582,7 @@ int omx_ir_solve(omx_solver *s, const float *LU, int64_t lda, uint64_t seed, con
             double *vj = V + (int64_t)j * n;
             hipLaunchKernelGGL(k_zero_d, dim3(vgrid(n)), dim3(256), 0, st, n, w);
             OMX_LAUNCH_CHECK();
-            RC(omx_otf_gemv_f64(n, seed, diag, 1.0, vj, w, s->gemv_ws.p, s->gemv_ws.bytes, st));
+            RC(system_gemv(s, sys, 1.0, vj, w, st));
             RC(omx_trsv_lower_unit_f32f64(n, b, LU, lda, w, s->trsv_ws.p, s->trsv_ws.bytes, st));
             RC(omx_trsv_upper_f32f64(n, b, LU, lda, w, s->trsv_ws.p, s->trsv_ws.bytes, st));
             // classical Gram-Schmidt, twice

@@ -1,17 +1,22 @@
 // omxp_main.cpp -- command-line driver with the reference's surface
 // (main.cpp:85-252, :273-736): positional N B [P Q] and the flags -solv,
-// -alt, -log, -progress, -init, plus -prec, -ir, -seed, -maxit, -restart,
-// -device.  One GPU, 1x1 grid.  The timed region is LU + IR, generation
+// -alt, -log, -progress, -init, plus -prec (fp16 | fp8 | learned), -policy,
+// -ir, -seed, -maxit, -restart, -device.  One GPU, 1x1 grid.  With
+// -prec learned the MLP policy picks each step's Schur precision; if the
+// refinement then fails, the system is generated again and factored in FP16
+// (inside the timed region).  The timed region is LU + IR, generation
 // excluded (main.cpp:614-657); the rate is (2/3 n^3 + 3/2 n^2) / time
 // (main.cpp:663).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -26,12 +31,13 @@ struct Opts {
     double inner_tol = 1e-10;
     unsigned long long seed = 42;
     std::string prec = "fp16";
+    std::string policy;
 };
 
 void usage() {
     std::printf(
         "usage: omxp N B [P Q] [-solv 0|1] [-alt 0|1|2] [-log 0|1] [-progress k] [-init 1]\n"
-        "            [-prec fp16|fp8] [-ir classic|gmres] [-seed s] [-maxit k]\n"
+        "            [-prec fp16|fp8|learned] [-policy file] [-ir classic|gmres] [-seed s] [-maxit k]\n"
         "            [-restart m] [-device d]\n");
 }
 
@@ -60,6 +66,7 @@ bool parse(int argc, char **argv, Opts &o) {
         else if (!std::strcmp(a, "-init")) o.init = std::atoi(next(a));
         else if (!std::strcmp(a, "-skip")) o.skip = std::atoi(next(a));
         else if (!std::strcmp(a, "-prec")) o.prec = next(a);
+        else if (!std::strcmp(a, "-policy")) o.policy = next(a);
         else if (!std::strcmp(a, "-ir")) o.ir = !std::strcmp(next(a), "gmres") ? 1 : 0;
         else if (!std::strcmp(a, "-seed")) o.seed = std::strtoull(next(a), nullptr, 10);
         else if (!std::strcmp(a, "-maxit")) o.maxit = std::atoi(next(a));
@@ -112,7 +119,28 @@ int main(int argc, char **argv) {
         if (nf == 0) nf = 1;
         o.n = nf * o.b;
     }
-    const bool fp8 = (o.prec == "fp8");
+    if (o.prec != "fp16" && o.prec != "fp8" && o.prec != "learned") {
+        std::fprintf(stderr, "-prec must be fp16, fp8 or learned\n");
+        return 2;
+    }
+    const bool learned = (o.prec == "learned");
+    const bool fp8 = (o.prec == "fp8") || learned;
+    std::vector<float> params;
+    if (learned) {
+        if (o.policy.empty()) {
+            // next to the package: <pkg>/bin/omxp -> <pkg>/data/sched_mlp.f32
+            std::string exe = argv[0];
+            const size_t slash = exe.rfind('/');
+            const std::string dir = (slash == std::string::npos) ? std::string(".") : exe.substr(0, slash);
+            o.policy = dir + "/../data/sched_mlp.f32";
+        }
+        std::ifstream f(o.policy, std::ios::binary);
+        params.resize(OMX_MLP_NPARAMS);
+        if (!f.read(reinterpret_cast<char *>(params.data()), OMX_MLP_NPARAMS * sizeof(float))) {
+            std::fprintf(stderr, "cannot read %d parameters from %s\n", OMX_MLP_NPARAMS, o.policy.c_str());
+            return 2;
+        }
+    }
     CK(hipSetDevice(o.device));
     const long n = o.n, b = o.b, lda = n;
     std::printf("===============================================================================\n");
@@ -120,7 +148,8 @@ int main(int argc, char **argv) {
     std::printf("===============================================================================\n");
     std::printf("\t%s\n", omx_version());
     std::printf("\tn=%ld b=%ld 1x1 grid, alt=%d, Schur precision %s, IR %s (maxit %d)\n", n, b, o.alt,
-                fp8 ? "FP8 e4m3 (pow2 tile scale)" : "FP16", o.ir ? "GMRES" : "classical", o.maxit);
+                learned ? "learned FP8/FP16 per step" : (fp8 ? "FP8 e4m3 (pow2 tile scale)" : "FP16"),
+                o.ir ? "GMRES" : "classical", o.maxit);
 
     float *A = nullptr;
     double *d = nullptr, *rhs = nullptr, *x = nullptr;
@@ -142,10 +171,21 @@ int main(int argc, char **argv) {
     std::printf("\tInitialization = %.4f sec\n", now_s() - t);
 
     std::vector<int> sched(n / b, fp8 ? 1 : 0);
+    if (learned && omx_solver_set_policy(s, params.data()) != 0) {
+        std::fprintf(stderr, "cannot set the policy\n");
+        return 1;
+    }
     double tim[5] = {0};
     const double start = now_s();
-    int info = omx_lu_factor(s, A, lda, fp8 ? sched.data() : nullptr, tim);
+    int info = omx_lu_factor(s, A, lda, (fp8 && !learned) ? sched.data() : nullptr, tim);
     const double t_lu = now_s() - start;
+    if (learned) {
+        std::vector<int> used(n / b, 0);
+        omx_solver_last_schedule(s, used.data(), nullptr);
+        int n8 = 0;
+        for (int v : used) n8 += v > 0;
+        std::printf("learned schedule: %d of %ld steps in FP8\n", n8, (long)(n / b - 1));
+    }
     std::printf("LU factorization %.2f sec (trailing GEMM %.2f sec over %d launches, %.1f TFLOP/s)\n", t_lu,
                 tim[1] * 1e-3, (int)tim[2], tim[1] > 0 ? tim[3] / (tim[1] * 1e-3) * 1e-12 : 0.0);
     if (info) std::printf("LU reported a zero pivot at %d\n", info);
@@ -155,6 +195,16 @@ int main(int argc, char **argv) {
     std::vector<double> trace(2 * o.maxit, 0.0);
     int rc = omx_ir_solve(s, A, lda, o.seed, d, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, 16.0, 0, out,
                           trace.data());
+    if (learned && rc != 0) {
+        // fallback: the learned schedule's factors did not refine; FP16
+        std::printf("learned schedule did not converge: regenerating and factoring in FP16\n");
+        if (omx_matgen_f32(n, o.seed, A, lda, d, nullptr)) return 1;
+        CK(hipDeviceSynchronize());
+        std::fill(sched.begin(), sched.end(), 0);
+        info = omx_lu_factor(s, A, lda, sched.data(), tim);
+        rc = omx_ir_solve(s, A, lda, o.seed, d, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, 16.0, 0, out,
+                          trace.data());
+    }
     const double stop = now_s();
     const int steps = (rc == 0) ? (int)out[2] + 1 : o.maxit;
     for (int i = 0; i < steps; ++i)
@@ -169,7 +219,7 @@ int main(int argc, char **argv) {
     std::printf("===================================\n");
     std::printf("Code   : omxp\n");
     std::printf("N      : %ld\nNB     : %ld\nP      : 1\nQ      : 1\n", n, b);
-    std::printf("PFACT  : Right\nDEPTH  : 1\nPREC   : %s\n", fp8 ? "fp8" : "fp16");
+    std::printf("PFACT  : Right\nDEPTH  : 1\nPREC   : %s\n", o.prec.c_str());
     std::printf("TIME(s): %.3f\nGFLOPS : %.1f\nRESID  : %20.15e\nHARNESS: %.9f\n", dur, flops / dur * 1e-9, resid,
                 harness);
     omx_solver_destroy(s);

@@ -1074,3 +1074,131 @@ int orc_gmres_ir(int64_t n, int64_t b, uint64_t seed, const float *LU,
     free(yv);
     return result;
 }
+
+/* ------------------------------------------------------------------------ */
+/* randsvd test matrices (new; no reference counterpart)                     */
+/* ------------------------------------------------------------------------ */
+static void randsvd_reflector(int64_t n, uint64_t seed, int t, double *w) {
+    double nrm2 = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        /* linear index n^2 + n + t*n + i = i + n*(n + 1 + t) */
+        w[i] = orc_lcg_value(orc_lcg_state(seed, (uint64_t)n, (uint64_t)i, (uint64_t)(n + 1 + t)));
+        nrm2 += w[i] * w[i];
+    }
+    const double inv = 1.0 / sqrt(nrm2);
+    for (int64_t i = 0; i < n; ++i) w[i] *= inv;
+}
+
+void orc_randsvd_dense(int64_t n, uint64_t seed, double kappa, int nrefl,
+                       double *A) {
+    double *w = (double *)malloc(sizeof(double) * (size_t)n);
+    double *t = (double *)malloc(sizeof(double) * (size_t)n);
+    memset(A, 0, sizeof(double) * (size_t)n * (size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+        A[i + i * n] = (n == 1) ? 1.0 : pow(kappa, -(double)i / (double)(n - 1));
+    /* right factors: A <- A (I - 2 w w^T), t = 2k-1 .. k */
+    for (int r = 2 * nrefl - 1; r >= nrefl; --r) {
+        randsvd_reflector(n, seed, r, w);
+        for (int64_t i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int64_t j = 0; j < n; ++j) s += A[i + j * n] * w[j];
+            t[i] = s;
+        }
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < n; ++i) A[i + j * n] -= 2.0 * t[i] * w[j];
+    }
+    /* left factors: A <- (I - 2 w w^T) A, t = k-1 .. 0 */
+    for (int r = nrefl - 1; r >= 0; --r) {
+        randsvd_reflector(n, seed, r, w);
+        for (int64_t j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (int64_t i = 0; i < n; ++i) s += w[i] * A[i + j * n];
+            t[j] = s;
+        }
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < n; ++i) A[i + j * n] -= 2.0 * w[i] * t[j];
+    }
+    free(w);
+    free(t);
+}
+
+/* ------------------------------------------------------------------------ */
+/* MLP 16-64-64-3 (new; the learned FP8/FP16 schedule)                        */
+/* ------------------------------------------------------------------------ */
+#define W1_OFF 0
+#define B1_OFF (W1_OFF + ORC_MLP_IN * ORC_MLP_H)
+#define W2_OFF (B1_OFF + ORC_MLP_H)
+#define B2_OFF (W2_OFF + ORC_MLP_H * ORC_MLP_H)
+#define W3_OFF (B2_OFF + ORC_MLP_H)
+#define B3_OFF (W3_OFF + ORC_MLP_H * ORC_MLP_OUT)
+
+static void mlp_fwd1(const float *p, const float *x, float *h1, float *h2, float *z) {
+    for (int j = 0; j < ORC_MLP_H; ++j) {
+        float a = p[B1_OFF + j];
+        for (int i = 0; i < ORC_MLP_IN; ++i) a += x[i] * p[W1_OFF + i * ORC_MLP_H + j];
+        h1[j] = a > 0.0f ? a : 0.0f;
+    }
+    for (int j = 0; j < ORC_MLP_H; ++j) {
+        float a = p[B2_OFF + j];
+        for (int i = 0; i < ORC_MLP_H; ++i) a += h1[i] * p[W2_OFF + i * ORC_MLP_H + j];
+        h2[j] = a > 0.0f ? a : 0.0f;
+    }
+    for (int j = 0; j < ORC_MLP_OUT; ++j) {
+        float a = p[B3_OFF + j];
+        for (int i = 0; i < ORC_MLP_H; ++i) a += h2[i] * p[W3_OFF + i * ORC_MLP_OUT + j];
+        z[j] = a;
+    }
+}
+
+void orc_mlp_forward(int64_t batch, const float *params, const float *X,
+                     float *H1, float *H2, float *Z) {
+    for (int64_t s = 0; s < batch; ++s)
+        mlp_fwd1(params, X + s * ORC_MLP_IN, H1 + s * ORC_MLP_H, H2 + s * ORC_MLP_H,
+                 Z + s * ORC_MLP_OUT);
+}
+
+float orc_mlp_backward(int64_t batch, const float *params, const float *X,
+                       const int32_t *y, const float *t, const float *m,
+                       float *grad) {
+    float h1[ORC_MLP_H], h2[ORC_MLP_H], z[ORC_MLP_OUT];
+    float dz[ORC_MLP_OUT], d2[ORC_MLP_H], d1[ORC_MLP_H];
+    double loss = 0.0;
+    memset(grad, 0, sizeof(float) * ORC_MLP_NPARAMS);
+    const float inv = 1.0f / (float)batch;
+    for (int64_t s = 0; s < batch; ++s) {
+        const float *x = X + s * ORC_MLP_IN;
+        mlp_fwd1(params, x, h1, h2, z);
+        /* softmax cross-entropy over (z0, z1) */
+        const float mx = z[0] > z[1] ? z[0] : z[1];
+        const float e0 = expf(z[0] - mx), e1 = expf(z[1] - mx);
+        const float lse = mx + logf(e0 + e1);
+        const float p0 = e0 / (e0 + e1), p1 = e1 / (e0 + e1);
+        const float diff = z[2] - t[s];
+        loss += (double)(lse - (y[s] ? z[1] : z[0])) + 0.5 * (double)(m[s] * diff * diff);
+        dz[0] = (p0 - (y[s] ? 0.0f : 1.0f)) * inv;
+        dz[1] = (p1 - (y[s] ? 1.0f : 0.0f)) * inv;
+        dz[2] = m[s] * diff * inv;
+        for (int j = 0; j < ORC_MLP_OUT; ++j) grad[B3_OFF + j] += dz[j];
+        for (int i = 0; i < ORC_MLP_H; ++i) {
+            float a = 0.0f;
+            for (int j = 0; j < ORC_MLP_OUT; ++j) {
+                grad[W3_OFF + i * ORC_MLP_OUT + j] += h2[i] * dz[j];
+                a += params[W3_OFF + i * ORC_MLP_OUT + j] * dz[j];
+            }
+            d2[i] = h2[i] > 0.0f ? a : 0.0f;
+        }
+        for (int j = 0; j < ORC_MLP_H; ++j) grad[B2_OFF + j] += d2[j];
+        for (int i = 0; i < ORC_MLP_H; ++i) {
+            float a = 0.0f;
+            for (int j = 0; j < ORC_MLP_H; ++j) {
+                grad[W2_OFF + i * ORC_MLP_H + j] += h1[i] * d2[j];
+                a += params[W2_OFF + i * ORC_MLP_H + j] * d2[j];
+            }
+            d1[i] = h1[i] > 0.0f ? a : 0.0f;
+        }
+        for (int j = 0; j < ORC_MLP_H; ++j) grad[B1_OFF + j] += d1[j];
+        for (int i = 0; i < ORC_MLP_IN; ++i)
+            for (int j = 0; j < ORC_MLP_H; ++j) grad[W1_OFF + i * ORC_MLP_H + j] += x[i] * d1[j];
+    }
+    return (float)(loss / (double)batch);
+}

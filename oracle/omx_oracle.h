@@ -107,6 +107,32 @@ int orc_gmres_ir(int64_t n, int64_t b, uint64_t seed, const float *LU,
                  double *resid, double *harness, double *trace,
                  int *inner_total);
 
+/* ---- randsvd test matrices (no reference counterpart) ------------------- */
+/* Dense FP64 A = H_0..H_{k-1} diag(sigma) H_{2k-1}..H_k (column-major,
+ * lda = n), sigma_i = kappa^(-i/(n-1)), reflector t from the LCG values at
+ * linear indices n^2 + n + t*n + i, normalised; k = nrefl.  O(n^2) per
+ * reflector: small n only. */
+void orc_randsvd_dense(int64_t n, uint64_t seed, double kappa, int nrefl,
+                       double *A);
+
+/* ---- MLP 16-64-64-3 precision scheduler (new) --------------------------- */
+/* Parameters, one flat FP32 array of ORC_MLP_NPARAMS:
+ *   W1[16][64] | b1[64] | W2[64][64] | b2[64] | W3[64][3] | b3[3]
+ * (W[i][j]: input i -> unit j).  Forward: h1 = relu(x W1 + b1),
+ * h2 = relu(h1 W2 + b2), z = h2 W3 + b3.  Loss per sample: cross-entropy of
+ * softmax(z0, z1) against the class y (0: FP16, 1: FP8) plus
+ * 0.5 * m * (z2 - t)^2 (scale head, mask m); the batch loss is the mean. */
+#define ORC_MLP_IN 16
+#define ORC_MLP_H 64
+#define ORC_MLP_OUT 3
+#define ORC_MLP_NPARAMS (16 * 64 + 64 + 64 * 64 + 64 + 64 * 3 + 3)
+void orc_mlp_forward(int64_t batch, const float *params, const float *X,
+                     float *H1, float *H2, float *Z);
+/* grad (ORC_MLP_NPARAMS) = d(mean loss)/d(params); returns the mean loss. */
+float orc_mlp_backward(int64_t batch, const float *params, const float *X,
+                       const int32_t *y, const float *t, const float *m,
+                       float *grad);
+
 #ifdef __cplusplus
 }
 #endif

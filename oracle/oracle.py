@@ -80,6 +80,9 @@ def lib():
                                          C.POINTER(dbl), C.POINTER(dbl), vp]),
             "orc_gmres_ir": (C.c_int, [i64, i64, u64, vp, i64, vp, vp, C.c_int, C.c_int, dbl, vp,
                                        C.POINTER(dbl), C.POINTER(dbl), vp, C.POINTER(C.c_int)]),
+            "orc_randsvd_dense": (None, [i64, u64, dbl, C.c_int, vp]),
+            "orc_mlp_forward": (None, [i64, vp, vp, vp, vp, vp]),
+            "orc_mlp_backward": (flt, [i64, vp, vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -341,6 +344,40 @@ def gmres_ir(LU, b, seed, diag, rhs_, maxit=20, restart=50, inner_tol=1e-10):
                             C.byref(r), C.byref(h), _p(trace), C.byref(inner))
     return dict(iters=int(it), x=x, resid=r.value, harness=h.value, inner=inner.value,
                 trace=trace.reshape(-1, 2)[: (it + 1 if it >= 0 else maxit)])
+
+
+def randsvd_dense(n, seed, kappa, nrefl=2):
+    """FP64 dense randsvd matrix (n x n numpy, C order rows = matrix rows)."""
+    A = np.empty((n, n), dtype=np.float64, order="F")
+    lib().orc_randsvd_dense(n, seed, float(kappa), nrefl, _p(A))
+    return A
+
+
+MLP_NPARAMS = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 3 + 3
+
+
+def mlp_forward(params, X):
+    """(H1, H2, Z) of the 16-64-64-3 MLP for a batch X (B x 16)."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    B = X.shape[0]
+    H1 = np.empty((B, 64), np.float32)
+    H2 = np.empty((B, 64), np.float32)
+    Z = np.empty((B, 3), np.float32)
+    lib().orc_mlp_forward(B, _p(params), _p(X), _p(H1), _p(H2), _p(Z))
+    return H1, H2, Z
+
+
+def mlp_backward(params, X, y, t, m):
+    """(mean loss, gradient) of the MLP loss."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.int32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    m = np.ascontiguousarray(m, dtype=np.float32)
+    g = np.empty(MLP_NPARAMS, np.float32)
+    loss = lib().orc_mlp_backward(X.shape[0], _p(params), _p(X), _p(y), _p(t), _p(m), _p(g))
+    return float(loss), g
 
 
 # ---------------------------------------------------------------------------

@@ -92,3 +92,41 @@ def test_trsv_matches_oracle(orc, n, b):
     zo = orc.trsv_upper(LU, b, yg)
     zg = host(xd)
     assert np.max(np.abs(zg - zo)) <= 1e-12 * np.max(np.abs(zo))
+
+
+@pytest.mark.parametrize("n,lda", [(5, 7), (256, 256), (300, 303), (777, 780), (1000, 1000), (2304, 2305)])
+def test_trsv_ragged_and_strided_match_lapack(n, lda):
+    """Orders that are not multiples of the 256-row solve block, leading
+    dimensions with and without 16-byte column alignment; against LAPACK's
+    FP64 triangular solve of the same FP32 factors."""
+    import scipy.linalg as sla
+    torch = torch_cuda()
+    print(f"trsv ragged n={n} lda={lda}", flush=True)
+    L = _L()
+    rng = np.random.default_rng(n)
+    F = (np.tril(rng.uniform(-1, 1, (n, n)) / n, -1) + np.triu(rng.uniform(-1, 1, (n, n)) / n, 1)
+         + np.diag(rng.uniform(1, 2, n))).astype(np.float32)
+    store = np.zeros((n, lda), np.float32)  # column j at store[j, :lda]
+    store[:, :n] = F.T
+    Ad = dev(store.ravel())
+    r = rng.uniform(-1, 1, n)
+    xd = dev(r)
+    assert L.omx_trsv_lower_unit_f32f64(n, n, ptr(Ad), lda, ptr(xd), None, 0, None) == 0
+    F64 = F.astype(np.float64)
+    y = sla.solve_triangular(np.tril(F64, -1) + np.eye(n), r, lower=True, unit_diagonal=True)
+    yg = host(xd)
+    assert np.max(np.abs(yg - y)) <= 1e-13 * np.max(np.abs(y))
+    assert L.omx_trsv_upper_f32f64(n, n, ptr(Ad), lda, ptr(xd), None, 0, None) == 0
+    z = sla.solve_triangular(np.triu(F64), yg, lower=False)
+    zg = host(xd)
+    assert np.max(np.abs(zg - z)) <= 1e-13 * np.max(np.abs(z))
+
+
+def test_trsv_argument_errors():
+    torch = torch_cuda()
+    L = _L()
+    a = empty(64, torch.float32)
+    x = empty(8, torch.float64)
+    assert L.omx_trsv_lower_unit_f32f64(8, 3, ptr(a), 8, ptr(x), None, 0, None) < 0   # n % b
+    assert L.omx_trsv_upper_f32f64(8, 8, ptr(a), 4, ptr(x), None, 0, None) < 0        # lda < n
+    assert L.omx_trsv_upper_f32f64(0, 8, ptr(a), 8, ptr(x), None, 0, None) < 0
