@@ -58,6 +58,11 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=10240, help="order of the CPU baseline sample")
     ap.add_argument("--cpu-b", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device copy first (the reference kernel for PMC byte counters)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="print the Schur GEMM traffic from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
+                         "counter_collection files of a --copy-calibration run, and exit")
     return ap.parse_args()
 
 
@@ -207,8 +212,53 @@ def pmc_traffic():
     return float(m.group(1)) if m else None
 
 
+def summarize_pmc(fetch_csv, write_csv, n, b):
+    """Schur GEMM HBM traffic per launch from two rocprofv3 PMC runs, each
+    counter scaled by what it reads for the 1 GiB copy of the same run."""
+    import csv
+
+    def rows(path, counter):
+        out = []
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                out.append((r["Kernel_Name"], float(r["Counter_Value"]) * 1024.0))
+        return out
+
+    copy_bytes = float(1 << 30)
+    res = {}
+    for path, counter in ((fetch_csv, "FETCH_SIZE"), (write_csv, "WRITE_SIZE")):
+        rs = rows(path, counter)
+        cal = [v for k, v in rs if "copyBuffer" in k]
+        if not cal:
+            raise SystemExit(f"{path}: no copy kernel to calibrate {counter}")
+        # the copy reads and writes 1 GiB each
+        factor = copy_bytes / cal[0]
+        g = [v * factor for k, v in rs if "k_gemm_f16<false" in k]
+        res[counter] = (factor, g)
+    nl = len(res["FETCH_SIZE"][1])
+    fetch = sum(res["FETCH_SIZE"][1])
+    write = sum(res["WRITE_SIZE"][1])
+    # algorithmic bytes of the trailing updates: FP32 C read + write and the
+    # two packed FP16 panels, per launch m = n - (k+1) b, k = 1 .. nb-2
+    nb = n // b
+    ms = [n - (k + 1) * b for k in range(1, nb) if n - (k + 1) * b > 0]
+    alg = sum(8.0 * m * m + 2 * 2.0 * m * b for m in ms)
+    print(f"kernel: k_gemm_f16<false,4> (trailing update) at n={n} b={b}")
+    print(f"launches: {nl} (algorithmic model: {len(ms)})")
+    print(f"fetch_calibration_factor: {res['FETCH_SIZE'][0]:.4f}  (bytes per counted byte, 1 GiB copy)")
+    print(f"write_calibration_factor: {res['WRITE_SIZE'][0]:.4f}")
+    print(f"fetch_bytes_total: {fetch:.4e}")
+    print(f"write_bytes_total: {write:.4e}")
+    print(f"algorithmic_bytes_total: {alg:.4e}  (8 m^2 for C + 4 m b for the panels)")
+    print(f"traffic_over_algorithmic: {(fetch + write) / alg:.3f}")
+    print(f"traffic_bytes_per_launch: {(fetch + write) / max(nl, 1):.6e}")
+
+
 def main():
     args = parse()
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.n, args.b)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         log("bench.py measures one MI355X: multi-GPU runs are not part of this build")
@@ -222,6 +272,14 @@ def main():
             policy = scheduler.load_params()
         except (OSError, ValueError) as e:
             log(f"no trained policy ({e}); measuring the plain method only")
+    if args.copy_calibration:
+        import torch
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copied 1 GiB device to device (PMC calibration)")
     log(f"n={n} b={b} alt={args.alt}: generating and solving on cuda:0")
     gs = GpuSolve(n, b, args.alt, fp8=False)
     res16, fwd16 = measure(gs, args, "plain fp16 + IR")

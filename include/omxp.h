@@ -216,6 +216,18 @@ int omx_pack_f8(int64_t rows, int64_t k, const float *src, int64_t rs,
                 int64_t ls, uint8_t *P, int32_t *exps, float *tile_max,
                 void *stream);
 
+/* Pack P = fp16(S * 2^e_s) with one power-of-two exponent per 256-row
+ * strip (k % 256 == 0): e_s is the largest with max|strip| * 2^e_s < 2^15,
+ * so no value overflows FP16 and small panels stay clear of its subnormal
+ * range; sexp receives e_s (omx_strips(rows) ints), tile_max the 256 x 256
+ * tile maxima (recomputed unless have_max).  Pow2 scaling is exact: where
+ * the plain cast neither overflows nor goes subnormal the GEMM result is the
+ * same.  No reference counterpart (its conversions ignore their `scale`
+ * argument, panel_trf.hpp:259, :306). */
+int omx_pack_f16_scaled(int64_t rows, int64_t k, const float *src,
+                        int64_t rs, int64_t ls, int have_max, uint16_t *P,
+                        int32_t *sexp, float *tile_max, void *stream);
+
 /* omx_pack_f8 with `headroom` (0..3) bits below the tile maximum: e is the
  * largest with max|tile| * 2^e <= 448 * 2^-headroom.  have_max = 1: tile_max
  * already holds the tile maxima (from omx_panel_tile_maxabs) and is not
@@ -243,6 +255,16 @@ int omx_gemm_f16_packed(int64_t m, int64_t n, int64_t k, float alpha,
                         const uint16_t *X, int64_t x_row0, const uint16_t *Y,
                         int64_t y_row0, float beta, float *C, int64_t ldc,
                         void *stream);
+
+/* omx_gemm_f16_packed for panels packed by omx_pack_f16_scaled: output
+ * tile (i, j) takes alpha * 2^-(xsexp[strip i] + ysexp[strip j]).  Both
+ * exponent arrays NULL: unscaled. */
+int omx_gemm_f16_packed_scaled(int64_t m, int64_t n, int64_t k, float alpha,
+                               const uint16_t *X, const int32_t *xsexp,
+                               int64_t x_row0, const uint16_t *Y,
+                               const int32_t *ysexp, int64_t y_row0,
+                               float beta, float *C, int64_t ldc,
+                               void *stream);
 
 /* As omx_gemm_f16_packed with FP8 e4m3 panels and their tile exponents
  * (k % 256 == 0); the exponents enter the MFMA as E8M0 block scales, so each
@@ -342,7 +364,8 @@ int omx_sched_features_f32(int64_t ntiles_l, const float *lmax,
  * ------------------------------------------------------------------------- */
 typedef struct omx_solver omx_solver;
 
-#define OMX_SOLVER_FP8 1 /* flag of omx_solver_create: also allocate FP8 panels */
+#define OMX_SOLVER_FP8 1     /* flag of omx_solver_create: also allocate FP8 panels */
+#define OMX_SOLVER_SCALE16 2 /* flag: FP16 Schur panels with per-strip pow2 scales */
 
 /* Create a solver for order n, block b (n % b == 0, b % 256 == 0), panel
  * method alt (0: FP32 TRSM, 1: FP16 GEMM with the TRSM-on-identity inverse,

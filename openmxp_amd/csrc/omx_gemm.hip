@@ -181,7 +181,8 @@ __device__ __forceinline__ void epilogue(floatx16 (&acc)[4][4], int64_t M, int64
 template <bool kBetaZero, int kNbuf>
 __global__ void __launch_bounds__(kThreads, 1)
 k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *__restrict__ X,
-           const char *__restrict__ Y, float *__restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
+           const char *__restrict__ Y, float *__restrict__ C, int64_t ldc, int tiles_m, int tiles_n,
+           const int32_t *__restrict__ xsexp, const int32_t *__restrict__ ysexp) {
     __shared__ __attribute__((aligned(1024))) char smem[kNbuf][2][16384];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -281,7 +282,9 @@ k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char 
     int st = 0;
     for (; st + kNbuf - 1 < nks; ++st) stage(st, std::true_type{});
     for (; st < nks; ++st) stage(st, std::false_type{});
-    epilogue<kBetaZero>(acc, M, N, alpha, beta, C, ldc, tm, tn, wm, wn, lane);
+    // per-strip power-of-two panel scales: the products carry 2^(ex + ey)
+    const float a = xsexp ? ldexpf(alpha, -(xsexp[tm] + ysexp[tn])) : alpha;
+    epilogue<kBetaZero>(acc, M, N, a, beta, C, ldc, tm, tn, wm, wn, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -422,24 +425,34 @@ int gemm_args(int64_t m, int64_t n, int64_t k, int64_t kstage, const void *X, in
 
 extern "C" {
 
-int omx_gemm_f16_packed(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X,
-                        int64_t x_row0, const uint16_t *Y, int64_t y_row0, float beta, float *C,
-                        int64_t ldc, void *stream) {
+int omx_gemm_f16_packed_scaled(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X,
+                               const int32_t *xsexp, int64_t x_row0, const uint16_t *Y, const int32_t *ysexp,
+                               int64_t y_row0, float beta, float *C, int64_t ldc, void *stream) {
     int rc = gemm_args(m, n, k, OMX_KSTAGE16, X, x_row0, Y, y_row0, C, ldc);
     if (rc) return rc;
+    if ((xsexp == nullptr) != (ysexp == nullptr)) return OMX_ERR_ARG;
     if (m == 0 || n == 0) return OMX_OK;
     const int tiles_m = (int)omx_cdiv(m, OMX_STRIP), tiles_n = (int)omx_cdiv(n, OMX_STRIP);
     const char *xp = reinterpret_cast<const char *>(X) + (x_row0 / OMX_STRIP) * OMX_STRIP * k * 2;
     const char *yp = reinterpret_cast<const char *>(Y) + (y_row0 / OMX_STRIP) * OMX_STRIP * k * 2;
+    const int32_t *xe = xsexp ? xsexp + x_row0 / OMX_STRIP : nullptr;
+    const int32_t *ye = ysexp ? ysexp + y_row0 / OMX_STRIP : nullptr;
     dim3 grid((unsigned)(tiles_m * tiles_n));
     if (beta == 0.0f)
         hipLaunchKernelGGL((k_gemm_f16<true, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
-                           beta, xp, yp, C, ldc, tiles_m, tiles_n);
+                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye);
     else
         hipLaunchKernelGGL((k_gemm_f16<false, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
-                           beta, xp, yp, C, ldc, tiles_m, tiles_n);
+                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye);
     OMX_LAUNCH_CHECK();
     return OMX_OK;
+}
+
+int omx_gemm_f16_packed(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X,
+                        int64_t x_row0, const uint16_t *Y, int64_t y_row0, float beta, float *C,
+                        int64_t ldc, void *stream) {
+    return omx_gemm_f16_packed_scaled(m, n, k, alpha, X, nullptr, x_row0, Y, nullptr, y_row0, beta, C, ldc,
+                                      stream);
 }
 
 int omx_gemm_f8_packed(int64_t m, int64_t n, int64_t k, float alpha, const uint8_t *X,

@@ -59,7 +59,8 @@ __device__ __forceinline__ void load_tile(float (*tile)[OMX_STRIP + 1], const fl
 // One workgroup per (row strip, 64-column stage): 32 KB of packed output.
 __global__ void __launch_bounds__(kPackThreads)
 k_pack_f16(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, int64_t ls,
-           int mask, _Float16 *__restrict__ P, float *__restrict__ chunk_max) {
+           int mask, _Float16 *__restrict__ P, float *__restrict__ chunk_max,
+           const int32_t *__restrict__ sexp) {
     __shared__ float tile[kTileK][OMX_STRIP + 1];
     __shared__ float red[kPackThreads / 64];
     const int64_t ks = blockIdx.x;   // K-stage
@@ -69,6 +70,7 @@ k_pack_f16(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, int6
     __syncthreads();
     _Float16 *out = P + (rsb * (K >> 6) + ks) * 16384;
     const int t = threadIdx.x;
+    const float scale = sexp ? ldexpf(1.0f, sexp[rsb]) : 1.0f;
     float mx = 0.0f;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -82,7 +84,7 @@ k_pack_f16(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, int6
         for (int e = 0; e < 8; ++e) {
             const float v = tile[kb + e][rl];
             mx = fmaxf(mx, fabsf(v));
-            h[e] = (_Float16)v;
+            h[e] = (_Float16)(v * scale);
         }
         *reinterpret_cast<half8 *>(out + oidx * 8) = h;
     }
@@ -136,6 +138,24 @@ __global__ void k_tile_exps(int64_t n, const float *__restrict__ tmax, int headr
         const int e = pow2_exp_448(tmax[i]);
         exps[i] = (tmax[i] > 0.0f) ? (e - headroom < -100 ? -100 : e - headroom) : 0;
     }
+}
+
+// Per-strip FP16 scale: the largest e with max|strip| * 2^e <= 2^15 (half the
+// FP16 range, clear of overflow), clamped to [-100, 100]; 0 for a zero strip.
+__global__ void k_strip_exps16(int64_t nstrips, int64_t KT, const float *__restrict__ tmax,
+                               int32_t *__restrict__ sexp) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nstrips) return;
+    float m = 0.0f;
+    for (int64_t q = 0; q < KT; ++q) m = fmaxf(m, tmax[s * KT + q]);
+    int e = 0;
+    if (m > 0.0f && m < __builtin_inff()) {
+        int em;
+        frexpf(m, &em);  // m in [2^(em-1), 2^em)
+        e = 15 - em;     // m * 2^e < 2^15
+        e = e > 100 ? 100 : (e < -100 ? -100 : e);
+    }
+    sexp[s] = e;
 }
 
 // FP8 pack: one workgroup per (row strip, 64-column half of a 128 stage).
@@ -196,7 +216,29 @@ int omx_pack_f16(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t 
     if (ls == 1 && (rs % 4 || (reinterpret_cast<uintptr_t>(src) & 15))) return OMX_ERR_ARG;
     dim3 grid((unsigned)(k / kTileK), (unsigned)omx_strips(rows));
     hipLaunchKernelGGL(k_pack_f16, grid, dim3(kPackThreads), 0, omx_stream(stream), rows, k, src, rs,
-                       ls, mask, reinterpret_cast<_Float16 *>(P), chunk_max);
+                       ls, mask, reinterpret_cast<_Float16 *>(P), chunk_max, nullptr);
+    OMX_LAUNCH_CHECK();
+    return OMX_OK;
+}
+
+int omx_pack_f16_scaled(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t ls, int have_max,
+                        uint16_t *P, int32_t *sexp, float *tile_max, void *stream) {
+    if (rows <= 0 || k <= 0 || k % OMX_SCALE_TILE || !src || !P || !sexp || !tile_max) return OMX_ERR_ARG;
+    if (!((rs == 1 && ls >= rows) || (ls == 1 && rs >= k))) return OMX_ERR_ARG;
+    if (ls == 1 && (rs % 4 || (reinterpret_cast<uintptr_t>(src) & 15))) return OMX_ERR_ARG;
+    hipStream_t st = omx_stream(stream);
+    const int64_t strips = omx_strips(rows), KT = omx_cdiv(k, OMX_SCALE_TILE);
+    if (!have_max) {
+        hipLaunchKernelGGL(k_tile_maxabs, dim3((unsigned)KT, (unsigned)strips), dim3(kPackThreads), 0, st,
+                           rows, k, src, rs, ls, tile_max);
+        OMX_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_strip_exps16, dim3((unsigned)omx_cdiv(strips, 256)), dim3(256), 0, st, strips, KT, tile_max,
+                       sexp);
+    OMX_LAUNCH_CHECK();
+    dim3 grid((unsigned)(k / kTileK), (unsigned)strips);
+    hipLaunchKernelGGL(k_pack_f16, grid, dim3(kPackThreads), 0, st, rows, k, src, rs, ls, (int)OMX_MASK_NONE,
+                       reinterpret_cast<_Float16 *>(P), nullptr, sexp);
     OMX_LAUNCH_CHECK();
     return OMX_OK;
 }

@@ -57,6 +57,8 @@ struct omx_solver {
     DevBuf Lp[2], Rp[2], Tp;            // FP16 panels and the -alt temporary
     DevBuf L8[2], R8[2], Le[2], Re[2];  // FP8 panels and exponents
     DevBuf L8max, R8max;                // tile max-abs scratch
+    int scale16 = 0;                    // FP16 Schur panels with per-strip scales
+    DevBuf Lse[2], Rse[2];              // their strip exponents
     DevBuf Wup, Wlo, Wup_h, Wlo_h;      // -alt inverses (FP32, packed FP16)
     DevBuf cmaxL, cmaxR;                // chunk max-abs of the FP16 packs
     DevBuf info;
@@ -178,6 +180,7 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
     s->nb = n / b;
     s->alt = alt;
     s->fp8 = (flags & OMX_SOLVER_FP8) ? 1 : 0;
+    s->scale16 = (flags & OMX_SOLVER_SCALE16) ? 1 : 0;
     int rc = OMX_OK;
     auto fail = [&](int code) {
         omx_solver_destroy(s);
@@ -208,9 +211,15 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
                 return fail(rc);
         }
     }
-    if (s->fp8) {
+    if (s->scale16)
+        for (int i = 0; i < 2; ++i)
+            if ((rc = dalloc(s->Lse[i], omx_strips(rows) * 4)) || (rc = dalloc(s->Rse[i], omx_strips(rows) * 4)))
+                return fail(rc);
+    if (s->fp8 || s->scale16) {
         const size_t ne = omx_panel_exps_count(rows, b);
         if ((rc = dalloc(s->L8max, ne * 4)) || (rc = dalloc(s->R8max, ne * 4))) return fail(rc);
+    }
+    if (s->fp8) {
         if ((rc = dalloc(s->pol, OMX_MLP_NPARAMS * 4)) || (rc = dalloc(s->feat, (size_t)nb * OMX_MLP_NFEATURES * 4)) ||
             (rc = dalloc(s->mlp_h, 2 * 64 * 4)) || (rc = dalloc(s->zdev, 16)))
             return fail(rc);
@@ -246,7 +255,8 @@ int omx_solver_destroy(omx_solver *s) {
                       &s->R8[1], &s->Le[0], &s->Le[1], &s->Re[0], &s->Re[1], &s->L8max, &s->R8max,
                       &s->Wup, &s->Wlo, &s->Wup_h, &s->Wlo_h, &s->cmaxL, &s->cmaxR, &s->info, &s->r,
                       &s->w, &s->gemv_ws, &s->trsv_ws, &s->nrm, &s->V, &s->hv, &s->dots,
-                      &s->pol, &s->feat, &s->mlp_h, &s->zdev};
+                      &s->pol, &s->feat, &s->mlp_h, &s->zdev, &s->Lse[0], &s->Lse[1], &s->Rse[0],
+                      &s->Rse[1]};
     for (DevBuf *d : bufs) dfree(*d);
     if (s->ev_dec) (void)hipEventDestroy(s->ev_dec);
     if (s->zhost) (void)hipHostFree(s->zhost);
@@ -265,6 +275,11 @@ static int schur(omx_solver *s, int use8, int cur, int64_t m, int64_t nn, int64_
         return omx_gemm_f8_packed(m, nn, b, -1.0f, static_cast<uint8_t *>(s->L8[cur].p),
                                   static_cast<int32_t *>(s->Le[cur].p), lr0, static_cast<uint8_t *>(s->R8[cur].p),
                                   static_cast<int32_t *>(s->Re[cur].p), rr0, 1.0f, C, ldc, st);
+    if (s->scale16)
+        return omx_gemm_f16_packed_scaled(m, nn, b, -1.0f, static_cast<uint16_t *>(s->Lp[cur].p),
+                                          static_cast<int32_t *>(s->Lse[cur].p), lr0,
+                                          static_cast<uint16_t *>(s->Rp[cur].p),
+                                          static_cast<int32_t *>(s->Rse[cur].p), rr0, 1.0f, C, ldc, st);
     return omx_gemm_f16_packed(m, nn, b, -1.0f, static_cast<uint16_t *>(s->Lp[cur].p), lr0,
                                static_cast<uint16_t *>(s->Rp[cur].p), rr0, 1.0f, C, ldc, st);
 }
@@ -278,6 +293,14 @@ static int pack_panels(omx_solver *s, int nxt, int use8, int headroom, int have_
                           static_cast<int32_t *>(s->Le[nxt].p), static_cast<float *>(s->L8max.p), st));
         RC(omx_pack_f8_ex(rest, b, Arow, lda, 1, headroom, have_max, static_cast<uint8_t *>(s->R8[nxt].p),
                           static_cast<int32_t *>(s->Re[nxt].p), static_cast<float *>(s->R8max.p), st));
+        return OMX_OK;
+    }
+    if (s->scale16) {
+        // the tile maxima are there when the step's statistics were taken
+        RC(omx_pack_f16_scaled(rest, b, Acol, 1, lda, have_max, static_cast<uint16_t *>(s->Lp[nxt].p),
+                               static_cast<int32_t *>(s->Lse[nxt].p), static_cast<float *>(s->L8max.p), st));
+        RC(omx_pack_f16_scaled(rest, b, Arow, lda, 1, have_max, static_cast<uint16_t *>(s->Rp[nxt].p),
+                               static_cast<int32_t *>(s->Rse[nxt].p), static_cast<float *>(s->R8max.p), st));
         return OMX_OK;
     }
     RC(omx_pack_f16(rest, b, Acol, 1, lda, OMX_MASK_NONE, static_cast<uint16_t *>(s->Lp[nxt].p),

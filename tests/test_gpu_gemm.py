@@ -207,3 +207,84 @@ def test_gemm_f8_schur(orc):
     # and the FP8 result approximates the FP32 product to e4m3 accuracy
     full = C0.T.astype(np.float64) - Xs.astype(np.float64) @ Ys.astype(np.float64).T
     assert np.max(np.abs(got - full)) < 0.1 * np.max(np.abs(Xs.astype(np.float64) @ Ys.T.astype(np.float64)))
+
+
+def _pack16_scaled_dev(S):
+    """Column panel S (R x K, column-major on the device) packed with
+    per-strip scales; returns (packed, exponents) device tensors."""
+    torch = torch_cuda()
+    L = _L()
+    R, K = S.shape
+    sd = dev(np.asfortranarray(S).ravel(order="F"))
+    P = empty(L.omx_panel_bytes(R, K, 0) // 2, torch.int16)
+    se = empty(-(-R // 256), torch.int32)
+    tm = empty(L.omx_panel_exps_count(R, K), torch.float32)
+    assert L.omx_pack_f16_scaled(R, K, ptr(sd), 1, R, 0, ptr(P), ptr(se), ptr(tm), None) == 0
+    return P, se
+
+
+def test_pack_f16_scaled_bitexact():
+    """Strip exponent = largest e with max|strip| 2^e < 2^15; values
+    fp16(S 2^e) in the packed layout."""
+    rng = np.random.default_rng(12)
+    R, K = 700, 512
+    S = (rng.standard_normal((R, K)) * 10.0 ** rng.uniform(-9, 7, (R, 1))).astype(np.float32)
+    P, se = _pack16_scaled_dev(S)
+    e = host(se)
+    scaled = np.zeros_like(S)
+    for s in range(e.size):
+        blk = S[s * 256:(s + 1) * 256]
+        m = float(np.abs(blk).max())
+        assert m * 2.0 ** e[s] < 2.0**15 <= m * 2.0 ** (e[s] + 1)
+        scaled[s * 256:(s + 1) * 256] = np.ldexp(blk, e[s])
+    assert np.array_equal(host(P).view(np.uint16), pack16(scaled.astype(np.float16)))
+
+
+def test_gemm_f16_scaled_equals_unscaled_in_range():
+    """Power-of-two scaling is exact: for panels whose plain FP16 cast
+    neither overflows nor goes subnormal, the scaled GEMM is bit-identical
+    to the plain one."""
+    torch = torch_cuda()
+    L = _L()
+    rng = np.random.default_rng(13)
+    m, n, k = 768, 512, 512
+    X = (rng.uniform(-1, 1, (m, k)) * 4.0 ** rng.integers(-3, 4, (m, 1))).astype(np.float16).astype(np.float32)
+    Y = rng.uniform(-1, 1, (n, k)).astype(np.float16).astype(np.float32)
+    XP, xe = _pack16_scaled_dev(X)
+    YP, ye = _pack16_scaled_dev(Y)
+    Xd = dev(pack16(X.astype(np.float16)).view(np.int16))
+    Yd = dev(pack16(Y.astype(np.float16)).view(np.int16))
+    C0 = rng.uniform(-1, 1, (n, m)).astype(np.float32)
+    C1, C2 = dev(C0.ravel()), dev(C0.ravel())
+    assert L.omx_gemm_f16_packed(m, n, k, -1.0, ptr(Xd), 0, ptr(Yd), 0, 1.0, ptr(C1), m, None) == 0
+    assert L.omx_gemm_f16_packed_scaled(m, n, k, -1.0, ptr(XP), ptr(xe), 0, ptr(YP), ptr(ye), 0, 1.0, ptr(C2), m,
+                                        None) == 0
+    assert np.array_equal(host(C1), host(C2))
+
+
+def test_gemm_f16_scaled_out_of_fp16_range():
+    """Panels of magnitude 1e7 and 1e-9: the plain FP16 cast overflows /
+    flushes, the scaled path stays within the FP32 accumulation bound of the
+    exact product of its (scaled-rounded) operands."""
+    torch = torch_cuda()
+    L = _L()
+    rng = np.random.default_rng(14)
+    m, n, k = 512, 256, 256
+    X = (rng.uniform(-1, 1, (m, k)) * 1e7).astype(np.float32)
+    Y = (rng.uniform(-1, 1, (n, k)) * 1e-9).astype(np.float32)
+    XP, xe = _pack16_scaled_dev(X)
+    YP, ye = _pack16_scaled_dev(Y)
+    Cd = dev(np.zeros(n * m, np.float32))
+    assert L.omx_gemm_f16_packed_scaled(m, n, k, 1.0, ptr(XP), ptr(xe), 0, ptr(YP), ptr(ye), 0, 0.0, ptr(Cd), m,
+                                        None) == 0
+    got = host(Cd).reshape(n, m).T.astype(np.float64)
+    ex, ey = host(xe), host(ye)
+    Xr = np.concatenate([np.ldexp(np.ldexp(X[s * 256:(s + 1) * 256], ex[s]).astype(np.float16).astype(np.float64),
+                                  -ex[s]) for s in range(ex.size)])
+    Yr = np.concatenate([np.ldexp(np.ldexp(Y[s * 256:(s + 1) * 256], ey[s]).astype(np.float16).astype(np.float64),
+                                  -ey[s]) for s in range(ey.size)])
+    exact = Xr @ Yr.T
+    bound = (k + 2) * 2.0**-24 * (np.abs(Xr) @ np.abs(Yr).T)
+    assert np.all(np.isfinite(got))
+    assert np.all(np.abs(got - exact) <= bound)
+    assert np.max(np.abs(exact - X.astype(np.float64) @ Y.astype(np.float64).T)) <= 2.0**-10 * np.max(np.abs(exact))

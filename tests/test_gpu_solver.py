@@ -179,3 +179,21 @@ def test_fp8_schedule_converges():
     S.close()
     print(f"  fp8: steps {int(out[2])} harness {out[1]:.4f} inner {int(out[3])}", flush=True)
     assert info == 0 and rc == 0 and out[1] < 16.0
+
+
+def test_scaled_fp16_lu_equals_plain_in_range():
+    """OMX_SOLVER_SCALE16 at n = 1536: every panel entry is a normal FP16
+    number there, so the per-strip scaling changes nothing."""
+    torch = torch_cuda()
+    n, b, seed = 1536, 256, 3
+    outs = []
+    for flags in (0, 2):
+        A, d, rhs = _generate(n, seed, n)
+        L = _L()
+        h = C.c_void_p()
+        assert L.omx_solver_create(n, b, 1, flags, C.byref(h)) == 0
+        tim = np.zeros(5)
+        assert L.omx_lu_factor(h, ptr(A), n, None, tim.ctypes.data) == 0
+        L.omx_solver_destroy(h)
+        outs.append(host(A))
+    assert np.array_equal(outs[0], outs[1])
