@@ -98,6 +98,13 @@ def lib():
             raise OmxError(
                 f"{LIB_PATH} is missing: build it with `python -m openmxp_amd.build` "
                 "(the HIP path has no fallback)")
+        # torch (device memory and streams in the tests and the benchmark)
+        # ships its own HIP runtime; importing it first makes this library
+        # bind to the same runtime instead of loading a second one
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             if not hasattr(L, name):

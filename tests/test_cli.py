@@ -1,0 +1,64 @@
+"""The omxp command line: the reference's surface (N B [P Q], -solv, -alt,
+-log, ...) and its argument errors, which are decided before any GPU call
+(CPU tests), plus small end-to-end runs on the GPU."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+OMXP = os.path.join(ROOT, "openmxp_amd", "bin", "omxp")
+
+
+def run(*args, timeout=300):
+    if not os.path.exists(OMXP):
+        pytest.skip("omxp not built")
+    return subprocess.run([OMXP, *map(str, args)], capture_output=True, text=True, timeout=timeout)
+
+
+def test_help():
+    r = run("-h")
+    assert r.returncode == 0 and "usage: omxp N B" in r.stdout
+
+
+@pytest.mark.parametrize("args", [("1024", "256", "2", "2"), ("1024", "256", "-skip", "1"),
+                                  ("1024", "256", "-prec", "bf16"), ("1024", "100"),
+                                  ("1024", "256", "-alt", "3"),
+                                  ("1024", "256", "-prec", "learned", "-policy", "/nonexistent/p.f32")])
+def test_argument_errors_exit_2(args):
+    r = run(*args)
+    assert r.returncode == 2, (r.stdout, r.stderr)
+
+
+def _harness(out):
+    for line in out.splitlines():
+        if line.startswith("HARNESS:"):
+            return float(line.split(":")[1])
+    raise AssertionError(out)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("extra", [(), ("-alt", "0"), ("-alt", "2", "-ir", "gmres"), ("-prec", "fp8", "-ir", "gmres")])
+def test_solve_small(extra):
+    from tests.gpu_util import torch_cuda
+    torch_cuda()
+    r = run(2048, 256, *extra)
+    print(r.stdout[-600:], flush=True)
+    assert r.returncode == 0, r.stderr
+    assert _harness(r.stdout) < 16.0
+
+
+@pytest.mark.gpu
+def test_learned_mode_with_a_constant_policy(tmp_path):
+    from tests.gpu_util import torch_cuda
+    torch_cuda()
+    p = np.zeros(5443, np.float32)
+    p[-3:] = [0.0, 1.0, 0.0]  # always FP8
+    f = tmp_path / "p.f32"
+    p.astype("<f4").tofile(f)
+    r = run(2048, 256, "-prec", "learned", "-policy", f, "-ir", "gmres")
+    print(r.stdout[-600:], flush=True)
+    assert r.returncode == 0, r.stderr
+    assert "learned schedule: 7 of 7 steps in FP8" in r.stdout
+    assert _harness(r.stdout) < 16.0

@@ -85,12 +85,14 @@ def test_lowrank_gemv_matches_dense(orc):
 def test_lu_gmres_ir_on_randsvd(orc, kappa, mode):
     """No-pivot LU (FP16, FP16 with per-strip scales, or FP8 Schur) +
     GMRES-IR on a randsvd system.
-    kappa <= 1e4: harness < 16 and the forward error against LAPACK's FP64
-    solve within the conditioning bound.  kappa = 1e6 is beyond a
+    FP16, kappa <= 1e4: harness < 16 and the forward error against LAPACK's
+    FP64 solve within the conditioning bound.  kappa = 1e6 is beyond a
     low-precision no-pivot factorisation (kappa * u16 ~ 500: the late pivots,
     ~1e-6, drown in the FP16 rounding of the Schur update and the factors
-    overflow): the solve must then report non-convergence, never a false
-    success."""
+    overflow), and FP8 (3 mantissa bits) is beyond it on these
+    non-dominant matrices at any kappa (the oracle's FP8 factors of the
+    kappa = 100 case have a backward error ~35 |A|): the solve must then
+    report non-convergence, never a false success."""
     torch = torch_cuda()
     L = _L()
     n, b = 2048, 256
@@ -113,7 +115,7 @@ def test_lu_gmres_ir_on_randsvd(orc, kappa, mode):
                                out.ctypes.data, trace.ctypes.data)
     L.omx_solver_destroy(h)
     print(f"  info {info} rc {rc} steps {int(out[2])} inner {int(out[3])} harness {out[1]:.4g}", flush=True)
-    if kappa <= 1e4:
+    if kappa <= 1e4 and not fp8:
         assert info == 0 and rc == 0 and out[1] < 16.0
     if rc != 0:
         assert rc == -4 and out[2] == -1  # OMX_ERR_NOCONV, no corrections claimed

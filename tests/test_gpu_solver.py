@@ -181,9 +181,11 @@ def test_fp8_schedule_converges():
     assert info == 0 and rc == 0 and out[1] < 16.0
 
 
-def test_scaled_fp16_lu_equals_plain_in_range():
-    """OMX_SOLVER_SCALE16 at n = 1536: every panel entry is a normal FP16
-    number there, so the per-strip scaling changes nothing."""
+def test_scaled_fp16_lu_matches_plain():
+    """OMX_SOLVER_SCALE16 at n = 1536: the scaled factors differ from the
+    plain ones only where a plain FP16 panel entry is subnormal (|L| below
+    2^-14 here); they agree to the FP16 tolerance and their backward error is
+    no worse."""
     torch = torch_cuda()
     n, b, seed = 1536, 256, 3
     outs = []
@@ -195,5 +197,15 @@ def test_scaled_fp16_lu_equals_plain_in_range():
         tim = np.zeros(5)
         assert L.omx_lu_factor(h, ptr(A), n, None, tim.ctypes.data) == 0
         L.omx_solver_destroy(h)
-        outs.append(host(A))
-    assert np.array_equal(outs[0], outs[1])
+        outs.append(colmajor_to_numpy(A, n, n).astype(np.float64))
+    from oracle import matgen_full
+    A0, _ = matgen_full(n, seed)
+    A0 = A0.astype(np.float64)
+    berr = []
+    for F in outs:
+        Lf = np.tril(F, -1) + np.eye(n)
+        berr.append(np.abs(Lf @ np.triu(F) - A0).max())
+    low = np.tril(np.ones((n, n), dtype=bool), -1)
+    assert np.abs(outs[0] - outs[1])[low].max() <= 2.0**-8 * np.abs(outs[0][low]).max()
+    print(f"  backward error plain {berr[0]:.3e} scaled {berr[1]:.3e}", flush=True)
+    assert berr[1] <= 1.05 * berr[0]
