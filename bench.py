@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (the reference kernel for PMC byte counters)")
+    ap.add_argument("--gemm-sweep", action="store_true",
+                    help="run only the FP16 trailing-update GEMMs of one factorisation (shapes m = n-(k+1)b) "
+                         "on the default stream, for counter collection")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the Schur GEMM traffic from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
                          "counter_collection files of a --copy-calibration run, and exit")
@@ -254,8 +257,48 @@ def summarize_pmc(fetch_csv, write_csv, n, b):
     print(f"traffic_bytes_per_launch: {(fetch + write) / max(nl, 1):.6e}")
 
 
+def gemm_sweep(n, b, seed):
+    """The 78 FP16 trailing updates of one factorisation, one launch each,
+    on the null stream, with packed panels of the right size and the
+    generated matrix as C; prints the event-timed rate."""
+    import torch
+    from openmxp_amd import _lib
+    L = _lib.lib()
+    A = torch.empty(n * n, dtype=torch.float32, device="cuda")
+    d = torch.empty(n, dtype=torch.float64, device="cuda")
+    assert L.omx_matgen_f32(n, seed, A.data_ptr(), n, d.data_ptr(), None) == 0
+    pb = L.omx_panel_bytes(n - b, b, 0)
+    X = (torch.rand(pb // 2, device="cuda") - 0.5).half()
+    Y = (torch.rand(pb // 2, device="cuda") - 0.5).half()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    flops = 0.0
+    e0.record()
+    for k in range(1, n // b - 1):
+        m = n - (k + 1) * b
+        off = (k + 1) * b * (n + 1)
+        assert L.omx_gemm_f16_packed(m, m, b, -1.0, X.data_ptr(), 0, Y.data_ptr(), 0, 1.0,
+                                     A.data_ptr() + 4 * off, n, None) == 0
+        flops += 2.0 * m * m * b
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3
+    log(f"gemm sweep n={n} b={b}: {n // b - 2} launches, {t * 1e3:.1f} ms, {flops / t * 1e-12:.1f} TFLOP/s")
+
+
 def main():
     args = parse()
+    if args.gemm_sweep:
+        if args.copy_calibration:
+            import torch
+            src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            del src, dst
+            log("copied 1 GiB device to device (PMC calibration)")
+        gemm_sweep(args.n, args.b, args.seed)
+        return
     if args.summarize_pmc:
         summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.n, args.b)
         return
