@@ -236,6 +236,24 @@ int omx_pack_f8_ex(int64_t rows, int64_t k, const float *src, int64_t rs,
                    int64_t ls, int headroom, int have_max, uint8_t *P,
                    int32_t *exps, float *tile_max, void *stream);
 
+/* K windows: a packed panel of ldk columns can hold several k-column
+ * panels side by side (columns [k0, k0 + k) of it).  omx_pack_f16_window /
+ * omx_pack_f8_window write one panel into such a window (FP16: plain RNE,
+ * or with per-strip scales as omx_pack_f16_scaled when sexp is given; FP8:
+ * as omx_pack_f8_ex, exps laid out as [strip][ldk/256], tile_max
+ * panel-local); the _window GEMMs multiply the windows [x_k0, x_k0 + k) and
+ * [y_k0, y_k0 + k).  This lets the LU apply two consecutive panels to the
+ * trailing matrix in one product (k = 2b).  k0 and ldk are multiples of 64
+ * (FP16) or 256 (FP8). */
+int omx_pack_f16_window(int64_t rows, int64_t k, const float *src,
+                        int64_t rs, int64_t ls, int have_max, uint16_t *P,
+                        int32_t *sexp, float *tile_max, int64_t ldk,
+                        int64_t k0, void *stream);
+int omx_pack_f8_window(int64_t rows, int64_t k, const float *src, int64_t rs,
+                       int64_t ls, int headroom, int have_max, uint8_t *P,
+                       int32_t *exps, int64_t ldk, int64_t k0,
+                       float *tile_max, void *stream);
+
 /* max |S| of each 256 x 256 tile of the panel S (k % 256 == 0),
  * omx_panel_exps_count(rows, k) floats, strip major.  No reference
  * counterpart (feeds the FP8 scales and the precision scheduler). */
@@ -265,6 +283,23 @@ int omx_gemm_f16_packed_scaled(int64_t m, int64_t n, int64_t k, float alpha,
                                const int32_t *ysexp, int64_t y_row0,
                                float beta, float *C, int64_t ldc,
                                void *stream);
+
+/* omx_gemm_f16_packed_scaled on K windows of wider packed panels. */
+int omx_gemm_f16_packed_window(int64_t m, int64_t n, int64_t k, float alpha,
+                               const uint16_t *X, const int32_t *xsexp,
+                               int64_t x_row0, int64_t x_ldk, int64_t x_k0,
+                               const uint16_t *Y, const int32_t *ysexp,
+                               int64_t y_row0, int64_t y_ldk, int64_t y_k0,
+                               float beta, float *C, int64_t ldc,
+                               void *stream);
+
+/* omx_gemm_f8_packed on K windows of wider packed panels. */
+int omx_gemm_f8_packed_window(int64_t m, int64_t n, int64_t k, float alpha,
+                              const uint8_t *X, const int32_t *xexp,
+                              int64_t x_row0, int64_t x_ldk, int64_t x_k0,
+                              const uint8_t *Y, const int32_t *yexp,
+                              int64_t y_row0, int64_t y_ldk, int64_t y_k0,
+                              float beta, float *C, int64_t ldc, void *stream);
 
 /* As omx_gemm_f16_packed with FP8 e4m3 panels and their tile exponents
  * (k % 256 == 0); the exponents enter the MFMA as E8M0 block scales, so each

@@ -48,9 +48,15 @@ SIGNATURES = {
     "omx_pack_f8": (i32, [i64, i64, vp, i64, i64, vp, vp, vp, vp]),
     "omx_pack_f16_scaled": (i32, [i64, i64, vp, i64, i64, i32, vp, vp, vp, vp]),
     "omx_pack_f8_ex": (i32, [i64, i64, vp, i64, i64, i32, i32, vp, vp, vp, vp]),
+    "omx_pack_f16_window": (i32, [i64, i64, vp, i64, i64, i32, vp, vp, vp, i64, i64, vp]),
+    "omx_pack_f8_window": (i32, [i64, i64, vp, i64, i64, i32, i32, vp, vp, i64, i64, vp, vp]),
     "omx_panel_tile_maxabs": (i32, [i64, i64, vp, i64, i64, vp, vp]),
     "omx_gemm_f16_packed": (i32, [i64, i64, i64, flt, vp, i64, vp, i64, flt, vp, i64, vp]),
     "omx_gemm_f16_packed_scaled": (i32, [i64, i64, i64, flt, vp, vp, i64, vp, vp, i64, flt, vp, i64, vp]),
+    "omx_gemm_f16_packed_window": (i32, [i64, i64, i64, flt, vp, vp, i64, i64, i64, vp, vp, i64, i64, i64, flt, vp,
+                                         i64, vp]),
+    "omx_gemm_f8_packed_window": (i32, [i64, i64, i64, flt, vp, vp, i64, i64, i64, vp, vp, i64, i64, i64, flt, vp,
+                                        i64, vp]),
     "omx_gemm_f8_packed": (i32, [i64, i64, i64, flt, vp, vp, i64, vp, vp, i64, flt, vp, i64, vp]),
     "omx_sgemm_f32": (i32, [i64, i64, i64, flt, vp, i64, vp, i64, flt, vp, i64, vp]),
     "omx_getrf_nopiv_f32": (i32, [i64, vp, i64, vp, vp]),

@@ -182,15 +182,18 @@ template <bool kBetaZero, int kNbuf>
 __global__ void __launch_bounds__(kThreads, 1)
 k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *__restrict__ X,
            const char *__restrict__ Y, float *__restrict__ C, int64_t ldc, int tiles_m, int tiles_n,
-           const int32_t *__restrict__ xsexp, const int32_t *__restrict__ ysexp) {
+           const int32_t *__restrict__ xsexp, const int32_t *__restrict__ ysexp, int64_t xstrip,
+           int64_t ystrip) {
     __shared__ __attribute__((aligned(1024))) char smem[kNbuf][2][16384];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     int tm, tn;
     tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const int nks = (int)(K / 32);
-    const char *xs = X + (int64_t)tm * (K / OMX_KSTAGE16) * kChunk;
-    const char *ys = Y + (int64_t)tn * (K / OMX_KSTAGE16) * kChunk;
+    // strips are xstrip / ystrip bytes apart (the packed panels may be wider
+    // than the K window this product uses)
+    const char *xs = X + (int64_t)tm * xstrip;
+    const char *ys = Y + (int64_t)tn * ystrip;
 
     auto issue = [&](int st) {
         const int64_t base = (int64_t)(st >> 1) * kChunk + (st & 1) * 2048;
@@ -296,18 +299,19 @@ template <bool kBetaZero, int kNbuf>
 __global__ void __launch_bounds__(kThreads, 1)
 k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *__restrict__ X,
           const int32_t *__restrict__ xexp, const char *__restrict__ Y, const int32_t *__restrict__ yexp,
-          float *__restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
+          float *__restrict__ C, int64_t ldc, int tiles_m, int tiles_n, int64_t xstrip, int64_t ystrip,
+          int64_t xet, int64_t yet) {
     __shared__ __attribute__((aligned(1024))) char smem[kNbuf][2][16384];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     int tm, tn;
     tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const int nks = (int)(K / 64);
-    const int64_t KT = K / OMX_SCALE_TILE;
-    const char *xs = X + (int64_t)tm * (K / OMX_KSTAGE8) * kChunk;
-    const char *ys = Y + (int64_t)tn * (K / OMX_KSTAGE8) * kChunk;
-    const int32_t *xe = xexp + (int64_t)tm * KT;
-    const int32_t *ye = yexp + (int64_t)tn * KT;
+    // strips xstrip / ystrip bytes apart; xet / yet exponents per strip
+    const char *xs = X + (int64_t)tm * xstrip;
+    const char *ys = Y + (int64_t)tn * ystrip;
+    const int32_t *xe = xexp + (int64_t)tm * xet;
+    const int32_t *ye = yexp + (int64_t)tn * yet;
 
     auto issue = [&](int st) {
         const int64_t base = (int64_t)(st >> 1) * kChunk + (st & 1) * 2048;
@@ -421,62 +425,87 @@ int gemm_args(int64_t m, int64_t n, int64_t k, int64_t kstage, const void *X, in
     return OMX_OK;
 }
 
+// the K window [k0, k0 + k) of a packed panel of ldk columns
+bool window_ok(int64_t k, int64_t ldk, int64_t k0, int64_t align) {
+    return ldk > 0 && ldk % align == 0 && k0 >= 0 && k0 % align == 0 && k0 + k <= ldk;
+}
+
 }  // namespace
 
 extern "C" {
 
-int omx_gemm_f16_packed_scaled(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X,
-                               const int32_t *xsexp, int64_t x_row0, const uint16_t *Y, const int32_t *ysexp,
-                               int64_t y_row0, float beta, float *C, int64_t ldc, void *stream) {
+int omx_gemm_f16_packed_window(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X,
+                               const int32_t *xsexp, int64_t x_row0, int64_t x_ldk, int64_t x_k0,
+                               const uint16_t *Y, const int32_t *ysexp, int64_t y_row0, int64_t y_ldk,
+                               int64_t y_k0, float beta, float *C, int64_t ldc, void *stream) {
     int rc = gemm_args(m, n, k, OMX_KSTAGE16, X, x_row0, Y, y_row0, C, ldc);
     if (rc) return rc;
     if ((xsexp == nullptr) != (ysexp == nullptr)) return OMX_ERR_ARG;
+    if (!window_ok(k, x_ldk, x_k0, OMX_KSTAGE16) || !window_ok(k, y_ldk, y_k0, OMX_KSTAGE16)) return OMX_ERR_ARG;
     if (m == 0 || n == 0) return OMX_OK;
     const int tiles_m = (int)omx_cdiv(m, OMX_STRIP), tiles_n = (int)omx_cdiv(n, OMX_STRIP);
-    const char *xp = reinterpret_cast<const char *>(X) + (x_row0 / OMX_STRIP) * OMX_STRIP * k * 2;
-    const char *yp = reinterpret_cast<const char *>(Y) + (y_row0 / OMX_STRIP) * OMX_STRIP * k * 2;
+    const int64_t xstrip = (x_ldk / OMX_KSTAGE16) * kChunk, ystrip = (y_ldk / OMX_KSTAGE16) * kChunk;
+    const char *xp = reinterpret_cast<const char *>(X) + (x_row0 / OMX_STRIP) * xstrip + (x_k0 / OMX_KSTAGE16) * kChunk;
+    const char *yp = reinterpret_cast<const char *>(Y) + (y_row0 / OMX_STRIP) * ystrip + (y_k0 / OMX_KSTAGE16) * kChunk;
     const int32_t *xe = xsexp ? xsexp + x_row0 / OMX_STRIP : nullptr;
     const int32_t *ye = ysexp ? ysexp + y_row0 / OMX_STRIP : nullptr;
     dim3 grid((unsigned)(tiles_m * tiles_n));
     if (beta == 0.0f)
         hipLaunchKernelGGL((k_gemm_f16<true, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
-                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye);
+                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye, xstrip, ystrip);
     else
         hipLaunchKernelGGL((k_gemm_f16<false, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
-                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye);
+                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye, xstrip, ystrip);
     OMX_LAUNCH_CHECK();
     return OMX_OK;
+}
+
+int omx_gemm_f16_packed_scaled(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X,
+                               const int32_t *xsexp, int64_t x_row0, const uint16_t *Y, const int32_t *ysexp,
+                               int64_t y_row0, float beta, float *C, int64_t ldc, void *stream) {
+    return omx_gemm_f16_packed_window(m, n, k, alpha, X, xsexp, x_row0, k, 0, Y, ysexp, y_row0, k, 0, beta, C, ldc,
+                                      stream);
 }
 
 int omx_gemm_f16_packed(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X,
                         int64_t x_row0, const uint16_t *Y, int64_t y_row0, float beta, float *C,
                         int64_t ldc, void *stream) {
-    return omx_gemm_f16_packed_scaled(m, n, k, alpha, X, nullptr, x_row0, Y, nullptr, y_row0, beta, C, ldc,
-                                      stream);
+    return omx_gemm_f16_packed_window(m, n, k, alpha, X, nullptr, x_row0, k, 0, Y, nullptr, y_row0, k, 0, beta, C,
+                                      ldc, stream);
+}
+
+int omx_gemm_f8_packed_window(int64_t m, int64_t n, int64_t k, float alpha, const uint8_t *X, const int32_t *xexp,
+                              int64_t x_row0, int64_t x_ldk, int64_t x_k0, const uint8_t *Y, const int32_t *yexp,
+                              int64_t y_row0, int64_t y_ldk, int64_t y_k0, float beta, float *C, int64_t ldc,
+                              void *stream) {
+    int rc = gemm_args(m, n, k, OMX_SCALE_TILE, X, x_row0, Y, y_row0, C, ldc);
+    if (rc) return rc;
+    if (!xexp || !yexp) return OMX_ERR_ARG;
+    if (!window_ok(k, x_ldk, x_k0, OMX_SCALE_TILE) || !window_ok(k, y_ldk, y_k0, OMX_SCALE_TILE)) return OMX_ERR_ARG;
+    if (m == 0 || n == 0) return OMX_OK;
+    const int tiles_m = (int)omx_cdiv(m, OMX_STRIP), tiles_n = (int)omx_cdiv(n, OMX_STRIP);
+    const int64_t xstrip = (x_ldk / OMX_KSTAGE8) * kChunk, ystrip = (y_ldk / OMX_KSTAGE8) * kChunk;
+    const int64_t xet = x_ldk / OMX_SCALE_TILE, yet = y_ldk / OMX_SCALE_TILE;
+    const char *xp = reinterpret_cast<const char *>(X) + (x_row0 / OMX_STRIP) * xstrip + (x_k0 / OMX_KSTAGE8) * kChunk;
+    const char *yp = reinterpret_cast<const char *>(Y) + (y_row0 / OMX_STRIP) * ystrip + (y_k0 / OMX_KSTAGE8) * kChunk;
+    const int32_t *xe = xexp + (x_row0 / OMX_STRIP) * xet + x_k0 / OMX_SCALE_TILE;
+    const int32_t *ye = yexp + (y_row0 / OMX_STRIP) * yet + y_k0 / OMX_SCALE_TILE;
+    dim3 grid((unsigned)(tiles_m * tiles_n));
+    if (beta == 0.0f)
+        hipLaunchKernelGGL((k_gemm_f8<true, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
+                           beta, xp, xe, yp, ye, C, ldc, tiles_m, tiles_n, xstrip, ystrip, xet, yet);
+    else
+        hipLaunchKernelGGL((k_gemm_f8<false, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
+                           beta, xp, xe, yp, ye, C, ldc, tiles_m, tiles_n, xstrip, ystrip, xet, yet);
+    OMX_LAUNCH_CHECK();
+    return OMX_OK;
 }
 
 int omx_gemm_f8_packed(int64_t m, int64_t n, int64_t k, float alpha, const uint8_t *X,
                        const int32_t *xexp, int64_t x_row0, const uint8_t *Y, const int32_t *yexp,
                        int64_t y_row0, float beta, float *C, int64_t ldc, void *stream) {
-    int rc = gemm_args(m, n, k, OMX_SCALE_TILE, X, x_row0, Y, y_row0, C, ldc);
-    if (rc) return rc;
-    if (!xexp || !yexp) return OMX_ERR_ARG;
-    if (m == 0 || n == 0) return OMX_OK;
-    const int tiles_m = (int)omx_cdiv(m, OMX_STRIP), tiles_n = (int)omx_cdiv(n, OMX_STRIP);
-    const int64_t KT = k / OMX_SCALE_TILE;
-    const char *xp = reinterpret_cast<const char *>(X) + (x_row0 / OMX_STRIP) * OMX_STRIP * k;
-    const char *yp = reinterpret_cast<const char *>(Y) + (y_row0 / OMX_STRIP) * OMX_STRIP * k;
-    const int32_t *xe = xexp + (x_row0 / OMX_STRIP) * KT;
-    const int32_t *ye = yexp + (y_row0 / OMX_STRIP) * KT;
-    dim3 grid((unsigned)(tiles_m * tiles_n));
-    if (beta == 0.0f)
-        hipLaunchKernelGGL((k_gemm_f8<true, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
-                           beta, xp, xe, yp, ye, C, ldc, tiles_m, tiles_n);
-    else
-        hipLaunchKernelGGL((k_gemm_f8<false, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
-                           beta, xp, xe, yp, ye, C, ldc, tiles_m, tiles_n);
-    OMX_LAUNCH_CHECK();
-    return OMX_OK;
+    return omx_gemm_f8_packed_window(m, n, k, alpha, X, xexp, x_row0, k, 0, Y, yexp, y_row0, k, 0, beta, C, ldc,
+                                     stream);
 }
 
 }  // extern "C"

@@ -60,7 +60,7 @@ __device__ __forceinline__ void load_tile(float (*tile)[OMX_STRIP + 1], const fl
 __global__ void __launch_bounds__(kPackThreads)
 k_pack_f16(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, int64_t ls,
            int mask, _Float16 *__restrict__ P, float *__restrict__ chunk_max,
-           const int32_t *__restrict__ sexp) {
+           const int32_t *__restrict__ sexp, int64_t pchunks, int64_t pk0) {
     __shared__ float tile[kTileK][OMX_STRIP + 1];
     __shared__ float red[kPackThreads / 64];
     const int64_t ks = blockIdx.x;   // K-stage
@@ -68,7 +68,9 @@ k_pack_f16(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, int6
     const int64_t r0 = rsb * OMX_STRIP, k0 = ks * kTileK;
     load_tile(tile, src, rs, ls, R, r0, k0, mask);
     __syncthreads();
-    _Float16 *out = P + (rsb * (K >> 6) + ks) * 16384;
+    // the packed panel has pchunks 64-column chunks per strip; this panel's
+    // columns start at chunk pk0
+    _Float16 *out = P + (rsb * pchunks + pk0 + ks) * 16384;
     const int t = threadIdx.x;
     const float scale = sexp ? ldexpf(1.0f, sexp[rsb]) : 1.0f;
     float mx = 0.0f;
@@ -132,11 +134,14 @@ __device__ __forceinline__ int pow2_exp_448(float maxabs) {
     return e > 100 ? 100 : (e < -100 ? -100 : e);
 }
 
-__global__ void k_tile_exps(int64_t n, const float *__restrict__ tmax, int headroom, int32_t *__restrict__ exps) {
+// exps[strip * ekt + ek0 + t] from tmax[strip * KT + t] (the exponent array
+// may span a wider packed panel than the tile maxima)
+__global__ void k_tile_exps(int64_t n, int64_t KT, const float *__restrict__ tmax, int headroom,
+                            int32_t *__restrict__ exps, int64_t ekt, int64_t ek0) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const int e = pow2_exp_448(tmax[i]);
-        exps[i] = (tmax[i] > 0.0f) ? (e - headroom < -100 ? -100 : e - headroom) : 0;
+        exps[(i / KT) * ekt + ek0 + i % KT] = (tmax[i] > 0.0f) ? (e - headroom < -100 ? -100 : e - headroom) : 0;
     }
 }
 
@@ -161,18 +166,19 @@ __global__ void k_strip_exps16(int64_t nstrips, int64_t KT, const float *__restr
 // FP8 pack: one workgroup per (row strip, 64-column half of a 128 stage).
 __global__ void __launch_bounds__(kPackThreads)
 k_pack_f8(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, int64_t ls,
-          const int32_t *__restrict__ exps, uint8_t *__restrict__ P) {
+          const int32_t *__restrict__ exps, uint8_t *__restrict__ P, int64_t pchunks, int64_t pk0,
+          int64_t ekt, int64_t ek0) {
     __shared__ float tile[kTileK][OMX_STRIP + 1];
     const int64_t k64 = blockIdx.x;  // 64-column block
     const int64_t rsb = blockIdx.y;
     const int64_t r0 = rsb * OMX_STRIP, k0 = k64 * kTileK;
     load_tile(tile, src, rs, ls, R, r0, k0, OMX_MASK_NONE);
     __syncthreads();
-    const int64_t KT = omx_cdiv_dev(K, OMX_SCALE_TILE);
-    const float scale = ldexpf(1.0f, exps[rsb * KT + k0 / OMX_SCALE_TILE]);
+    const float scale = ldexpf(1.0f, exps[rsb * ekt + ek0 + k0 / OMX_SCALE_TILE]);
     const int64_t ks = k64 >> 1;
     const int kk = (int)(k64 & 1);
-    uint8_t *chunk = P + (rsb * (K >> 7) + ks) * 32768;
+    // pchunks 128-column chunks per strip; this panel starts at chunk pk0
+    uint8_t *chunk = P + (rsb * pchunks + pk0 + ks) * 32768;
     const int t = threadIdx.x;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -216,7 +222,7 @@ int omx_pack_f16(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t 
     if (ls == 1 && (rs % 4 || (reinterpret_cast<uintptr_t>(src) & 15))) return OMX_ERR_ARG;
     dim3 grid((unsigned)(k / kTileK), (unsigned)omx_strips(rows));
     hipLaunchKernelGGL(k_pack_f16, grid, dim3(kPackThreads), 0, omx_stream(stream), rows, k, src, rs,
-                       ls, mask, reinterpret_cast<_Float16 *>(P), chunk_max, nullptr);
+                       ls, mask, reinterpret_cast<_Float16 *>(P), chunk_max, nullptr, k / 64, (int64_t)0);
     OMX_LAUNCH_CHECK();
     return OMX_OK;
 }
@@ -238,14 +244,15 @@ int omx_pack_f16_scaled(int64_t rows, int64_t k, const float *src, int64_t rs, i
     OMX_LAUNCH_CHECK();
     dim3 grid((unsigned)(k / kTileK), (unsigned)strips);
     hipLaunchKernelGGL(k_pack_f16, grid, dim3(kPackThreads), 0, st, rows, k, src, rs, ls, (int)OMX_MASK_NONE,
-                       reinterpret_cast<_Float16 *>(P), nullptr, sexp);
+                       reinterpret_cast<_Float16 *>(P), nullptr, sexp, k / 64, (int64_t)0);
     OMX_LAUNCH_CHECK();
     return OMX_OK;
 }
 
-int omx_pack_f8_ex(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t ls, int headroom, int have_max,
-                   uint8_t *P, int32_t *exps, float *tile_max, void *stream) {
+int omx_pack_f8_window(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t ls, int headroom, int have_max,
+                       uint8_t *P, int32_t *exps, int64_t ldk, int64_t k0, float *tile_max, void *stream) {
     if (rows <= 0 || k <= 0 || k % OMX_SCALE_TILE || !src || !P || !exps || !tile_max) return OMX_ERR_ARG;
+    if (ldk <= 0 || ldk % OMX_SCALE_TILE || k0 < 0 || k0 % OMX_SCALE_TILE || k0 + k > ldk) return OMX_ERR_ARG;
     if (headroom < 0 || headroom > 3) return OMX_ERR_ARG;
     if (!((rs == 1 && ls >= rows) || (ls == 1 && rs >= k))) return OMX_ERR_ARG;
     if (ls == 1 && (rs % 4 || (reinterpret_cast<uintptr_t>(src) & 15))) return OMX_ERR_ARG;
@@ -257,11 +264,44 @@ int omx_pack_f8_ex(int64_t rows, int64_t k, const float *src, int64_t rs, int64_
         OMX_LAUNCH_CHECK();
     }
     const int64_t nt = strips * KT;
-    hipLaunchKernelGGL(k_tile_exps, dim3((unsigned)omx_cdiv(nt, 256)), dim3(256), 0, st, nt, tile_max, headroom,
-                       exps);
+    const int64_t ekt = ldk / OMX_SCALE_TILE, ek0 = k0 / OMX_SCALE_TILE;
+    hipLaunchKernelGGL(k_tile_exps, dim3((unsigned)omx_cdiv(nt, 256)), dim3(256), 0, st, nt, KT, tile_max, headroom,
+                       exps, ekt, ek0);
     OMX_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_pack_f8, dim3((unsigned)(k / kTileK), (unsigned)strips), dim3(kPackThreads), 0,
-                       st, rows, k, src, rs, ls, exps, P);
+                       st, rows, k, src, rs, ls, exps, P, ldk / OMX_KSTAGE8, k0 / OMX_KSTAGE8, ekt, ek0);
+    OMX_LAUNCH_CHECK();
+    return OMX_OK;
+}
+
+int omx_pack_f8_ex(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t ls, int headroom, int have_max,
+                   uint8_t *P, int32_t *exps, float *tile_max, void *stream) {
+    return omx_pack_f8_window(rows, k, src, rs, ls, headroom, have_max, P, exps, k, 0, tile_max, stream);
+}
+
+int omx_pack_f16_window(int64_t rows, int64_t k, const float *src, int64_t rs, int64_t ls, int have_max,
+                        uint16_t *P, int32_t *sexp, float *tile_max, int64_t ldk, int64_t k0, void *stream) {
+    if (rows <= 0 || k <= 0 || k % OMX_KSTAGE16 || !src || !P) return OMX_ERR_ARG;
+    if (ldk <= 0 || ldk % OMX_KSTAGE16 || k0 < 0 || k0 % OMX_KSTAGE16 || k0 + k > ldk) return OMX_ERR_ARG;
+    if (sexp && (!tile_max || k % OMX_SCALE_TILE)) return OMX_ERR_ARG;
+    if (!((rs == 1 && ls >= rows) || (ls == 1 && rs >= k))) return OMX_ERR_ARG;
+    if (ls == 1 && (rs % 4 || (reinterpret_cast<uintptr_t>(src) & 15))) return OMX_ERR_ARG;
+    hipStream_t st = omx_stream(stream);
+    const int64_t strips = omx_strips(rows);
+    if (sexp) {
+        const int64_t KT = omx_cdiv(k, OMX_SCALE_TILE);
+        if (!have_max) {
+            hipLaunchKernelGGL(k_tile_maxabs, dim3((unsigned)KT, (unsigned)strips), dim3(kPackThreads), 0, st,
+                               rows, k, src, rs, ls, tile_max);
+            OMX_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_strip_exps16, dim3((unsigned)omx_cdiv(strips, 256)), dim3(256), 0, st, strips, KT,
+                           tile_max, sexp);
+        OMX_LAUNCH_CHECK();
+    }
+    dim3 grid((unsigned)(k / kTileK), (unsigned)strips);
+    hipLaunchKernelGGL(k_pack_f16, grid, dim3(kPackThreads), 0, st, rows, k, src, rs, ls, (int)OMX_MASK_NONE,
+                       reinterpret_cast<_Float16 *>(P), nullptr, sexp, ldk / 64, k0 / 64);
     OMX_LAUNCH_CHECK();
     return OMX_OK;
 }

@@ -3,19 +3,26 @@
 // and GMRES) on the GPU.
 //
 // LU (panel_lu_async_wGPU_gdirect_alt2, panel_trf_gdirect.hpp:593-1192, on a
-// 1x1 grid).  Step k (1 <= k < nb) of the reference does, with the panels
-// L_{k-1}, R_{k-1} of the previous step:
-//   (a) A_kk -= L_{k-1}[k] R_{k-1}[k]^T; getrf(A_kk)
-//   (c) A[k+1:,k] -= L_{k-1}[k+1:] R_{k-1}[k]^T; column panel by -alt; L_k
-//   (d) A[k,k+1:] -= L_{k-1}[k] R_{k-1}[k+1:]^T; row panel by -alt; R_k
-//   (e) A[k+1:,k+1:] -= L_{k-1}[k+1:] R_{k-1}[k+1:]^T      (the bulk)
-// Here (a)-(d) run on the critical stream and (e) on the bulk stream.
-// crit(k) waits for bulk(k-1) (its blocks were in bulk(k-1)'s region) and
-// bulk(k) waits for crit(k-1) (it needs L_{k-1}), so crit(k) overlaps
-// bulk(k): the diagonal factorisation and the panels of step k hide behind
-// the trailing update, which the reference serialises on the NULL stream.
-// Every block still receives the updates of panels 0, 1, 2, ... in order,
-// each as one GEMM accumulated into FP32 storage, as in the reference.
+// 1x1 grid).  Panel i is the factored diagonal block A_ii with its column
+// panel A[i+1:, i] and row panel A[i, i+1:], packed to FP16 or FP8 as L_i
+// and R_i.  Every block (I, J) with I, J > i receives -L_i[I] R_i[J]^T, in
+// the order i = 0, 1, 2, ...; the reference applies one panel per pass over
+// the trailing matrix.  Here the trailing matrix receives the panels in
+// pairs (2p, 2p+1): the FP32 trailing matrix is read and written once per
+// pair with one product of depth 2b instead of twice (its traffic, not the
+// MFMA work, bounds the update's epilogue).  The two panels of a pair share
+// one packed buffer (panel 2p in K columns [0, b), panel 2p+1 in [b, 2b)).
+// Pair p, with panels 2p and 2p+1 factored:
+//   bulk stream  A[2p+4:, 2p+4:] -= [L_2p L_2p+1] [R_2p R_2p+1]^T
+//   crit stream  block row/col 2p+2 -= pair p; factor panel 2p+2;
+//                block row/col 2p+3 -= pair p, then -= panel 2p+2;
+//                factor panel 2p+3
+// The crit stream waits for the bulk product of pair p-1 (it covered blocks
+// 2p+2 and 2p+3), the bulk product of pair p for the pair's two panels; so
+// the factorisation of the next pair's panels overlaps the trailing update
+// of the current pair.  A pair whose panels
+// were packed in different precisions (or in scaled FP16, whose scales are
+// per panel) is applied as two products of depth b.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,13 +61,12 @@ struct omx_solver {
     int fp8 = 0;
     hipStream_t crit = nullptr, bulk = nullptr;
     std::vector<hipEvent_t> ev_panel, ev_bulk, ev_t0, ev_t1;
-    DevBuf Lp[2], Rp[2], Tp;            // FP16 panels and the -alt temporary
-    DevBuf L8[2], R8[2], Le[2], Re[2];  // FP8 panels and exponents
+    DevBuf Lp[2], Rp[2], Tp;            // FP16 pair buffers (K = 2b) and the -alt temporary
+    DevBuf L8[2], R8[2], Le[2], Re[2];  // FP8 pair buffers and their exponents
     DevBuf L8max, R8max;                // tile max-abs scratch
     int scale16 = 0;                    // FP16 Schur panels with per-strip scales
-    DevBuf Lse[2], Rse[2];              // their strip exponents
+    DevBuf Lse[4], Rse[4];              // their strip exponents, per (pair buffer, window)
     DevBuf Wup, Wlo, Wup_h, Wlo_h;      // -alt inverses (FP32, packed FP16)
-    DevBuf cmaxL, cmaxR;                // chunk max-abs of the FP16 packs
     DevBuf info;
     // learned schedule: MLP parameters, per-step features, forward scratch
     DevBuf pol, feat, mlp_h, zdev;
@@ -68,6 +74,7 @@ struct omx_solver {
     hipEvent_t ev_dec = nullptr;
     float *zhost = nullptr;            // pinned: the step's 3 outputs
     std::vector<int> last_sched;       // per step: 0 FP16, 1 + headroom FP8
+    std::vector<int> prec;             // the current factorisation's choices
     std::vector<float> last_feat;      // per step: 16 features (FP8 solvers)
     // refinement
     DevBuf r, w, gemv_ws, trsv_ws, nrm, V, hv, dots;
@@ -200,19 +207,20 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
         if (hipEventCreate(&s->ev_t1[k]) != hipSuccess) return fail(OMX_ERR_HIP);
     }
     const int64_t rows = (n > b) ? n - b : b;
-    const size_t pb16 = omx_panel_bytes(rows, b, 0);
+    // pair buffers: packed rows from the pair's first panel row, K = 2b
+    const size_t pb16 = omx_panel_bytes(rows, 2 * b, 0);
     for (int i = 0; i < 2; ++i) {
         if ((rc = dalloc(s->Lp[i], pb16)) || (rc = dalloc(s->Rp[i], pb16))) return fail(rc);
         if (s->fp8) {
-            const size_t pb8 = omx_panel_bytes(rows, b, 1);
-            const size_t ne = omx_panel_exps_count(rows, b);
+            const size_t pb8 = omx_panel_bytes(rows, 2 * b, 1);
+            const size_t ne = omx_panel_exps_count(rows, 2 * b);
             if ((rc = dalloc(s->L8[i], pb8)) || (rc = dalloc(s->R8[i], pb8)) ||
                 (rc = dalloc(s->Le[i], ne * 4)) || (rc = dalloc(s->Re[i], ne * 4)))
                 return fail(rc);
         }
     }
     if (s->scale16)
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 4; ++i)
             if ((rc = dalloc(s->Lse[i], omx_strips(rows) * 4)) || (rc = dalloc(s->Rse[i], omx_strips(rows) * 4)))
                 return fail(rc);
     if (s->fp8 || s->scale16) {
@@ -227,14 +235,13 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
         if (hipHostMalloc(reinterpret_cast<void **>(&s->zhost), 16) != hipSuccess) return fail(OMX_ERR_HIP);
     }
     s->last_sched.assign(nb, 0);
+    s->prec.assign(nb, 0);
     s->last_feat.assign((size_t)nb * OMX_MLP_NFEATURES, 0.0f);
     if ((rc = dalloc(s->Tp, pb16))) return fail(rc);
     if ((rc = dalloc(s->Wup, (size_t)b * b * 4)) || (rc = dalloc(s->Wlo, (size_t)b * b * 4)) ||
         (rc = dalloc(s->Wup_h, omx_panel_bytes(b, b, 0))) || (rc = dalloc(s->Wlo_h, omx_panel_bytes(b, b, 0))))
         return fail(rc);
-    const size_t ncm = (size_t)omx_strips(rows) * (size_t)(b / OMX_KSTAGE16);
-    if ((rc = dalloc(s->cmaxL, ncm * 4)) || (rc = dalloc(s->cmaxR, ncm * 4))) return fail(rc);
-    if ((rc = dalloc(s->info, 16))) return fail(rc);
+    if ((rc = dalloc(s->info, (size_t)nb * sizeof(int) + 16))) return fail(rc);
     // refinement vectors
     if ((rc = dalloc(s->r, n * 8)) || (rc = dalloc(s->w, n * 8)) ||
         (rc = dalloc(s->gemv_ws, std::max(std::max<size_t>(omx_otf_gemv_workspace_bytes(n), 8), omx_lowrank_gemv_workspace_bytes(16)))) ||
@@ -253,10 +260,10 @@ int omx_solver_destroy(omx_solver *s) {
             if (e) (void)hipEventDestroy(e);
     DevBuf *bufs[] = {&s->Lp[0], &s->Lp[1], &s->Rp[0], &s->Rp[1], &s->Tp, &s->L8[0], &s->L8[1], &s->R8[0],
                       &s->R8[1], &s->Le[0], &s->Le[1], &s->Re[0], &s->Re[1], &s->L8max, &s->R8max,
-                      &s->Wup, &s->Wlo, &s->Wup_h, &s->Wlo_h, &s->cmaxL, &s->cmaxR, &s->info, &s->r,
+                      &s->Wup, &s->Wlo, &s->Wup_h, &s->Wlo_h, &s->info, &s->r,
                       &s->w, &s->gemv_ws, &s->trsv_ws, &s->nrm, &s->V, &s->hv, &s->dots,
-                      &s->pol, &s->feat, &s->mlp_h, &s->zdev, &s->Lse[0], &s->Lse[1], &s->Rse[0],
-                      &s->Rse[1]};
+                      &s->pol, &s->feat, &s->mlp_h, &s->zdev, &s->Lse[0], &s->Lse[1], &s->Lse[2],
+                      &s->Lse[3], &s->Rse[0], &s->Rse[1], &s->Rse[2], &s->Rse[3]};
     for (DevBuf *d : bufs) dfree(*d);
     if (s->ev_dec) (void)hipEventDestroy(s->ev_dec);
     if (s->zhost) (void)hipHostFree(s->zhost);
@@ -266,47 +273,88 @@ int omx_solver_destroy(omx_solver *s) {
     return OMX_OK;
 }
 
-// Schur product C -= Lpanel(rows lr0..) * Rpanel(rows rr0..)^T in the
-// precision the panels of that step were packed in.
-static int schur(omx_solver *s, int use8, int cur, int64_t m, int64_t nn, int64_t lr0, int64_t rr0, float *C,
-                 int64_t ldc, hipStream_t st) {
-    const int64_t b = s->b;
-    if (use8)
-        return omx_gemm_f8_packed(m, nn, b, -1.0f, static_cast<uint8_t *>(s->L8[cur].p),
-                                  static_cast<int32_t *>(s->Le[cur].p), lr0, static_cast<uint8_t *>(s->R8[cur].p),
-                                  static_cast<int32_t *>(s->Re[cur].p), rr0, 1.0f, C, ldc, st);
-    if (s->scale16)
-        return omx_gemm_f16_packed_scaled(m, nn, b, -1.0f, static_cast<uint16_t *>(s->Lp[cur].p),
-                                          static_cast<int32_t *>(s->Lse[cur].p), lr0,
-                                          static_cast<uint16_t *>(s->Rp[cur].p),
-                                          static_cast<int32_t *>(s->Rse[cur].p), rr0, 1.0f, C, ldc, st);
-    return omx_gemm_f16_packed(m, nn, b, -1.0f, static_cast<uint16_t *>(s->Lp[cur].p), lr0,
-                               static_cast<uint16_t *>(s->Rp[cur].p), rr0, 1.0f, C, ldc, st);
+// Packed coordinates of panel i: pair buffer (i/2) % 2, K window (i % 2) b,
+// packed row 0 = matrix row (2 (i/2) + 1) b, the first row of the pair's
+// first column panel.
+static inline int pbuf(int64_t i) { return (int)((i >> 1) & 1); }
+static inline int64_t porigin(int64_t i, int64_t b) { return ((i & ~(int64_t)1) + 1) * b; }
+static inline int64_t pwin(int64_t i, int64_t b) { return (i & 1) * b; }
+
+// A(r0:r0+m, c0:c0+nn) -= L_i R_i^T over K window [k0, k0 + k) of panel i's
+// pair buffer (k = 2b, k0 = 0: both panels of the pair).
+static int schur_window(omx_solver *s, int64_t i, int64_t k, int64_t k0, int64_t r0, int64_t c0, int64_t m,
+                        int64_t nn, float *A, int64_t lda, hipStream_t st) {
+    const int64_t b = s->b, ldk = 2 * b;
+    const int buf = pbuf(i);
+    const int64_t xr0 = r0 - porigin(i, b), yr0 = c0 - porigin(i, b);
+    float *C = A + r0 + c0 * lda;
+    if (s->prec[i])
+        return omx_gemm_f8_packed_window(m, nn, k, -1.0f, static_cast<uint8_t *>(s->L8[buf].p),
+                                         static_cast<int32_t *>(s->Le[buf].p), xr0, ldk, k0,
+                                         static_cast<uint8_t *>(s->R8[buf].p), static_cast<int32_t *>(s->Re[buf].p),
+                                         yr0, ldk, k0, 1.0f, C, lda, st);
+    const int32_t *xe = nullptr, *ye = nullptr;
+    if (s->scale16) {
+        xe = static_cast<int32_t *>(s->Lse[2 * buf + (i & 1)].p);
+        ye = static_cast<int32_t *>(s->Rse[2 * buf + (i & 1)].p);
+    }
+    return omx_gemm_f16_packed_window(m, nn, k, -1.0f, static_cast<uint16_t *>(s->Lp[buf].p), xe, xr0, ldk, k0,
+                                      static_cast<uint16_t *>(s->Rp[buf].p), ye, yr0, ldk, k0, 1.0f, C, lda, st);
 }
 
-// Pack the Schur panels of step k from the finished FP32 column / row panels.
-static int pack_panels(omx_solver *s, int nxt, int use8, int headroom, int have_max, int64_t rest,
-                       const float *Acol, const float *Arow, int64_t lda, hipStream_t st) {
+// A(r0.., c0..) (m x nn) -= the panels i0 .. i1 (i1 = i0 or i0 + 1), in
+// order; one product of depth 2b when they are a pair packed alike.
+static int apply_panels(omx_solver *s, int64_t i0, int64_t i1, int64_t r0, int64_t c0, int64_t m, int64_t nn,
+                        float *A, int64_t lda, hipStream_t st) {
+    if (m <= 0 || nn <= 0) return OMX_OK;
     const int64_t b = s->b;
-    if (use8) {
-        RC(omx_pack_f8_ex(rest, b, Acol, 1, lda, headroom, have_max, static_cast<uint8_t *>(s->L8[nxt].p),
-                          static_cast<int32_t *>(s->Le[nxt].p), static_cast<float *>(s->L8max.p), st));
-        RC(omx_pack_f8_ex(rest, b, Arow, lda, 1, headroom, have_max, static_cast<uint8_t *>(s->R8[nxt].p),
-                          static_cast<int32_t *>(s->Re[nxt].p), static_cast<float *>(s->R8max.p), st));
+    const bool fuse = i1 == i0 + 1 && (i0 & 1) == 0 && (s->prec[i0] > 0) == (s->prec[i1] > 0) &&
+                      !(s->scale16 && s->prec[i0] == 0);
+    if (fuse) return schur_window(s, i0, 2 * b, 0, r0, c0, m, nn, A, lda, st);
+    for (int64_t i = i0; i <= i1; ++i) RC(schur_window(s, i, b, pwin(i, b), r0, c0, m, nn, A, lda, st));
+    return OMX_OK;
+}
+
+// Block row/col t (A_tt, A[t+1:, t], A[t, t+1:]) -= panels i0 .. i1.
+static int update_block(omx_solver *s, int64_t t, int64_t i0, int64_t i1, float *A, int64_t lda, hipStream_t st) {
+    const int64_t n = s->n, b = s->b, tb = t * b;
+    RC(apply_panels(s, i0, i1, tb, tb, n - tb, b, A, lda, st));
+    RC(apply_panels(s, i0, i1, tb, tb + b, b, n - tb - b, A, lda, st));
+    s->crit_flops += 2.0 * (double)(i1 - i0 + 1) * b * b * (2.0 * (n - tb) - b);
+    return OMX_OK;
+}
+
+// Pack panel i (its finished FP32 column / row panels) into its pair buffer
+// window, in the precision s->prec[i].
+static int pack_panel(omx_solver *s, int64_t i, int have_max, int64_t rest, const float *Acol, const float *Arow,
+                      int64_t lda, hipStream_t st) {
+    const int64_t b = s->b, ldk = 2 * b;
+    const int buf = pbuf(i);
+    const int64_t rstrip = (pwin(i, b) + 0) / OMX_STRIP;  // packed rows start at (i % 2) b
+    const int64_t k0 = pwin(i, b);
+    if (s->prec[i]) {
+        const int h = s->prec[i] - 1;
+        const size_t sb8 = (size_t)(ldk / OMX_KSTAGE8) * 32768, ne = (size_t)(ldk / OMX_SCALE_TILE);
+        uint8_t *L8 = static_cast<uint8_t *>(s->L8[buf].p) + rstrip * sb8;
+        uint8_t *R8 = static_cast<uint8_t *>(s->R8[buf].p) + rstrip * sb8;
+        int32_t *Le = static_cast<int32_t *>(s->Le[buf].p) + rstrip * ne;
+        int32_t *Re = static_cast<int32_t *>(s->Re[buf].p) + rstrip * ne;
+        RC(omx_pack_f8_window(rest, b, Acol, 1, lda, h, have_max, L8, Le, ldk, k0, static_cast<float *>(s->L8max.p),
+                              st));
+        RC(omx_pack_f8_window(rest, b, Arow, lda, 1, h, have_max, R8, Re, ldk, k0, static_cast<float *>(s->R8max.p),
+                              st));
         return OMX_OK;
     }
+    const size_t sb16 = (size_t)(ldk / OMX_KSTAGE16) * 16384;  // uint16 per strip
+    uint16_t *Lp = static_cast<uint16_t *>(s->Lp[buf].p) + rstrip * sb16;
+    uint16_t *Rp = static_cast<uint16_t *>(s->Rp[buf].p) + rstrip * sb16;
+    int32_t *xe = nullptr, *ye = nullptr;
     if (s->scale16) {
-        // the tile maxima are there when the step's statistics were taken
-        RC(omx_pack_f16_scaled(rest, b, Acol, 1, lda, have_max, static_cast<uint16_t *>(s->Lp[nxt].p),
-                               static_cast<int32_t *>(s->Lse[nxt].p), static_cast<float *>(s->L8max.p), st));
-        RC(omx_pack_f16_scaled(rest, b, Arow, lda, 1, have_max, static_cast<uint16_t *>(s->Rp[nxt].p),
-                               static_cast<int32_t *>(s->Rse[nxt].p), static_cast<float *>(s->R8max.p), st));
-        return OMX_OK;
+        xe = static_cast<int32_t *>(s->Lse[2 * buf + (i & 1)].p) + rstrip;
+        ye = static_cast<int32_t *>(s->Rse[2 * buf + (i & 1)].p) + rstrip;
     }
-    RC(omx_pack_f16(rest, b, Acol, 1, lda, OMX_MASK_NONE, static_cast<uint16_t *>(s->Lp[nxt].p),
-                    static_cast<float *>(s->cmaxL.p), st));
-    RC(omx_pack_f16(rest, b, Arow, lda, 1, OMX_MASK_NONE, static_cast<uint16_t *>(s->Rp[nxt].p),
-                    static_cast<float *>(s->cmaxR.p), st));
+    RC(omx_pack_f16_window(rest, b, Acol, 1, lda, have_max, Lp, xe, static_cast<float *>(s->L8max.p), ldk, k0, st));
+    RC(omx_pack_f16_window(rest, b, Arow, lda, 1, have_max, Rp, ye, static_cast<float *>(s->R8max.p), ldk, k0, st));
     return OMX_OK;
 }
 
@@ -380,50 +428,34 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
     // sched given: fixed choices; else the policy if one is set; else FP16
     const bool learned = !sched && s->has_policy;
     const bool stats = s->fp8 != 0;
-    auto fixed_choice = [&](int64_t k) { return sched ? sched[k] : 0; };
-    int *info = static_cast<int *>(s->info.p);
+    int *info = static_cast<int *>(s->info.p);  // one slot per diagonal block
     uint16_t *Tp = static_cast<uint16_t *>(s->Tp.p);
     float *Wup = static_cast<float *>(s->Wup.p), *Wlo = static_cast<float *>(s->Wlo.p);
     uint16_t *Wup_h = static_cast<uint16_t *>(s->Wup_h.p), *Wlo_h = static_cast<uint16_t *>(s->Wlo_h.p);
     s->bulk_count = 0;
     s->bulk_flops = 0;
     s->crit_flops = 0;
+    std::fill(s->prec.begin(), s->prec.end(), 0);
     OMX_HIP_CHECK(hipDeviceSynchronize());
     const double t0 = wall_ms();
-
-    // step 0: getrf(A00) and the two FP32 TRSM panels
-    // (panel_trf_gdirect.hpp:699-737; panel_trf.hpp:188-246)
     if (stats) OMX_HIP_CHECK(hipMemsetAsync(s->feat.p, 0, s->feat.bytes, cs));
-    RC(omx_getrf_nopiv_f32(b, A, lda, info, cs));
-    int choice = fixed_choice(0);
-    if (nb > 1) {
-        const int64_t rest = n - b;
-        RC(omx_trsm_f32(OMX_TRSM_RUNN, rest, b, A, lda, A + b, lda, cs));
-        RC(omx_trsm_f32(OMX_TRSM_LLNU, b, rest, A, lda, A + b * lda, lda, cs));
-        if (stats) RC(step_features(s, 0, rest, A, A + b, A + b * lda, lda, 0, learned, cs));
-        if (learned) RC(policy_decision(s, &choice));
-        RC(pack_panels(s, 0, choice > 0, choice > 0 ? choice - 1 : 0, stats, rest, A + b, A + b * lda, lda, cs));
-    }
-    s->last_sched[0] = choice;
-    int use8_prev = choice > 0 ? 1 : 0;
-    OMX_HIP_CHECK(hipEventRecord(s->ev_panel[0], cs));
 
-    for (int64_t k = 1; k < nb; ++k) {
-        const int cur = (int)((k - 1) & 1), nxt = (int)(k & 1);
-        const int64_t kb = k * b;
-        const int64_t rest = n - kb - b;
-        float *Akk = A + kb + kb * lda;
-        // ---- critical path: diagonal block, column and row panels ----
-        if (k >= 2) OMX_HIP_CHECK(hipStreamWaitEvent(cs, s->ev_bulk[k - 1], 0));
-        RC(schur(s, use8_prev, cur, b, b, 0, 0, Akk, lda, cs));
-        RC(omx_getrf_nopiv_f32(b, Akk, lda, info, cs));
+    // Factor panel i on the crit stream: getrf of the (up to date) diagonal
+    // block, the column and row panels (FP32 TRSM at step 0 and with -alt 0,
+    // panel_trf.hpp:188-246; the FP16 inverse products with -alt 1/2,
+    // panel_trf_gdirect.hpp:73-183), the step's statistics and precision,
+    // and the pack into the pair buffer.
+    auto factor = [&](int64_t i) -> int {
+        const int64_t ib = i * b, rest = n - ib - b;
+        float *Akk = A + ib + ib * lda;
+        RC(omx_getrf_nopiv_f32(b, Akk, lda, info + i, cs));
+        int choice = 0;
         if (rest > 0) {
-            float *Acol = A + (kb + b) + kb * lda;
-            float *Arow = A + kb + (kb + b) * lda;
-            // column panel (:832-860)
-            RC(schur(s, use8_prev, cur, rest, b, b, 0, Acol, lda, cs));
-            if (s->alt == 0) {
+            float *Acol = A + (ib + b) + ib * lda;
+            float *Arow = A + ib + (ib + b) * lda;
+            if (i == 0 || s->alt == 0) {
                 RC(omx_trsm_f32(OMX_TRSM_RUNN, rest, b, Akk, lda, Acol, lda, cs));
+                RC(omx_trsm_f32(OMX_TRSM_LLNU, b, rest, Akk, lda, Arow, lda, cs));
             } else {
                 RC(omx_identity_f32(b, Wup, b, cs));
                 RC(omx_trsm_f32(OMX_TRSM_RUNN, b, b, Akk, lda, Wup, b, cs));
@@ -431,47 +463,64 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
                 RC(omx_pack_f16(b, b, Wup, b, 1, s->alt == 2 ? OMX_MASK_K_LE_R : OMX_MASK_NONE, Wup_h, nullptr, cs));
                 RC(omx_pack_f16(rest, b, Acol, 1, lda, OMX_MASK_NONE, Tp, nullptr, cs));
                 RC(omx_gemm_f16_packed(rest, b, b, 1.0f, Tp, 0, Wup_h, 0, 0.0f, Acol, lda, cs));
-            }
-            // row panel (:864-892)
-            RC(schur(s, use8_prev, cur, b, rest, 0, b, Arow, lda, cs));
-            if (s->alt == 0) {
-                RC(omx_trsm_f32(OMX_TRSM_LLNU, b, rest, Akk, lda, Arow, lda, cs));
-            } else {
                 RC(omx_identity_f32(b, Wlo, b, cs));
                 RC(omx_trsm_f32(OMX_TRSM_RLNU, b, b, Akk, lda, Wlo, b, cs));
                 // X(i, l) = W(i, l), strict lower + unit diagonal for -alt 2
-                RC(omx_pack_f16(b, b, Wlo, 1, b, s->alt == 2 ? OMX_MASK_R_GT_K_UNIT : OMX_MASK_NONE, Wlo_h, nullptr, cs));
+                RC(omx_pack_f16(b, b, Wlo, 1, b, s->alt == 2 ? OMX_MASK_R_GT_K_UNIT : OMX_MASK_NONE, Wlo_h, nullptr,
+                                cs));
                 RC(omx_pack_f16(rest, b, Arow, lda, 1, OMX_MASK_NONE, Tp, nullptr, cs));
                 RC(omx_gemm_f16_packed(b, rest, b, 1.0f, Wlo_h, 0, Tp, 0, 0.0f, Arow, lda, cs));
             }
-            if (stats) RC(step_features(s, k, rest, Akk, Acol, Arow, lda, use8_prev, learned, cs));
             s->crit_flops += 4.0 * rest * (double)b * b;
+            const int prev = (i > 0 && s->prec[i - 1] > 0) ? 1 : 0;
+            if (stats) RC(step_features(s, i, rest, Akk, Acol, Arow, lda, prev, learned, cs));
+            // with a policy the host waits here for the step's outputs; the
+            // current pair's trailing update is already queued on the bulk
+            // stream, so the GPU stays busy
+            if (learned) RC(policy_decision(s, &choice));
+            else if (sched) choice = sched[i];
+            s->prec[i] = choice;
+            RC(pack_panel(s, i, stats, rest, Acol, Arow, lda, cs));
         }
-        // ---- bulk: trailing update with the previous panels (:900-915) ----
-        if (rest > 0) {
-            OMX_HIP_CHECK(hipStreamWaitEvent(bs, s->ev_panel[k - 1], 0));
+        s->prec[i] = choice;
+        s->last_sched[i] = choice;
+        return OMX_OK;
+    };
+
+    // panels 0 and 1
+    RC(factor(0));
+    if (nb > 1) {
+        RC(update_block(s, 1, 0, 0, A, lda, cs));
+        RC(factor(1));
+    }
+    OMX_HIP_CHECK(hipEventRecord(s->ev_panel[0], cs));
+    for (int64_t p = 0; 2 * p + 2 < nb; ++p) {
+        const int64_t i0 = 2 * p, i1 = 2 * p + 1;
+        // ---- bulk: the pair's trailing update beyond blocks 2p+2, 2p+3 ----
+        const int64_t f = 2 * p + 4;
+        if (f < nb) {
+            const int64_t m = n - f * b;
+            OMX_HIP_CHECK(hipStreamWaitEvent(bs, s->ev_panel[p], 0));
             OMX_HIP_CHECK(hipEventRecord(s->ev_t0[s->bulk_count], bs));
-            RC(schur(s, use8_prev, cur, rest, rest, b, b, A + (kb + b) + (kb + b) * lda, lda, bs));
+            RC(apply_panels(s, i0, i1, f * b, f * b, m, m, A, lda, bs));
             OMX_HIP_CHECK(hipEventRecord(s->ev_t1[s->bulk_count], bs));
-            s->bulk_flops += 2.0 * (double)rest * rest * b;
+            s->bulk_flops += 4.0 * (double)m * m * b;
             ++s->bulk_count;
         }
-        OMX_HIP_CHECK(hipEventRecord(s->ev_bulk[k], bs));
-        // ---- this step's precision, then its Schur panels ----
-        // (with a policy the host waits here for the step's panels; the
-        // trailing update above is already queued, so the GPU stays busy)
-        choice = fixed_choice(k);
-        if (rest > 0) {
-            if (learned) RC(policy_decision(s, &choice));
-            float *Acol = A + (kb + b) + kb * lda;
-            float *Arow = A + kb + (kb + b) * lda;
-            RC(pack_panels(s, nxt, choice > 0, choice > 0 ? choice - 1 : 0, stats, rest, Acol, Arow, lda, cs));
-        } else {
-            choice = 0;
+        OMX_HIP_CHECK(hipEventRecord(s->ev_bulk[p], bs));
+        // ---- crit: the next pair's panels ----
+        // blocks 2p+2 and 2p+3 hold the previous pairs' updates once the
+        // previous bulk product is done
+        if (p >= 1) OMX_HIP_CHECK(hipStreamWaitEvent(cs, s->ev_bulk[p - 1], 0));
+        const int64_t t0b = 2 * p + 2, t1b = 2 * p + 3;
+        RC(update_block(s, t0b, i0, i1, A, lda, cs));
+        RC(factor(t0b));
+        if (t1b < nb) {
+            RC(update_block(s, t1b, i0, i1, A, lda, cs));
+            RC(update_block(s, t1b, t0b, t0b, A, lda, cs));
+            RC(factor(t1b));
         }
-        s->last_sched[k] = choice;
-        OMX_HIP_CHECK(hipEventRecord(s->ev_panel[k], cs));
-        use8_prev = choice > 0 ? 1 : 0;
+        OMX_HIP_CHECK(hipEventRecord(s->ev_panel[p + 1], cs));
     }
     OMX_HIP_CHECK(hipStreamSynchronize(cs));
     OMX_HIP_CHECK(hipStreamSynchronize(bs));
@@ -482,8 +531,8 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
         OMX_HIP_CHECK(hipEventElapsedTime(&ms, s->ev_t0[i], s->ev_t1[i]));
         s->bulk_ms += ms;
     }
-    int hinfo = 0;
-    OMX_HIP_CHECK(hipMemcpy(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> hinfo(nb, 0);
+    OMX_HIP_CHECK(hipMemcpy(hinfo.data(), info, sizeof(int) * nb, hipMemcpyDeviceToHost));
     if (timings) {
         timings[0] = s->t_lu_ms;
         timings[1] = s->bulk_ms;
@@ -491,7 +540,9 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
         timings[3] = s->bulk_flops;
         timings[4] = s->crit_flops;
     }
-    return (hinfo == 0x7fffffff) ? OMX_OK : hinfo;
+    for (int64_t i = 0; i < nb; ++i)
+        if (hinfo[i] != 0x7fffffff) return (int)(i * b + hinfo[i]);
+    return OMX_OK;
 }
 
 // Refinement.  method 0: classical IR (iterative_refinement.hpp:141-171);

@@ -288,3 +288,92 @@ def test_gemm_f16_scaled_out_of_fp16_range():
     assert np.all(np.isfinite(got))
     assert np.all(np.abs(got - exact) <= bound)
     assert np.max(np.abs(exact - X.astype(np.float64) @ Y.astype(np.float64).T)) <= 2.0**-10 * np.max(np.abs(exact))
+
+
+@pytest.mark.parametrize("fp8", [False, True])
+def test_window_pair_product_equals_two_products(orc, fp8):
+    """Two k-column panels packed side by side in a 2k-column buffer (the
+    second starting at packed row `shift`, as the LU packs a pair): the
+    product over both windows equals C - X1 Y1^T - X2 Y2^T within the FP32
+    accumulation bound, and the product over the second window alone is
+    bit-identical to the same panel packed on its own."""
+    torch = torch_cuda()
+    L = _L()
+    rng = np.random.default_rng(21 + fp8)
+    k, shift = 512, 256
+    m, n = 768, 512  # target rows/cols counted from packed row `shift`
+    R = shift + m
+    X1 = rng.uniform(-1, 1, (R, k)).astype(np.float32)
+    Y1 = rng.uniform(-1, 1, (shift + n, k)).astype(np.float32)
+    X2 = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    Y2 = rng.uniform(-1, 1, (n, k)).astype(np.float32)
+    ldk = 2 * k
+    fp = 1 if fp8 else 0
+    PX = empty(L.omx_panel_bytes(R, ldk, fp), torch.uint8)
+    PY = empty(L.omx_panel_bytes(shift + n, ldk, fp), torch.uint8)
+    ex = empty(L.omx_panel_exps_count(R, ldk), torch.int32)
+    ey = empty(L.omx_panel_exps_count(shift + n, ldk), torch.int32)
+    tmax = empty(L.omx_panel_exps_count(R, k) + 64, torch.float32)
+    strip_bytes = (ldk // (128 if fp8 else 64)) * 32768
+    ne = ldk // 256
+
+    def pack(S, P, E, row_shift, k0):
+        sd = dev(np.asfortranarray(S).ravel(order="F"))
+        base = ptr(P) + (row_shift // 256) * strip_bytes
+        if fp8:
+            eb = ptr(E) + 4 * (row_shift // 256) * ne
+            assert L.omx_pack_f8_window(S.shape[0], k, ptr(sd), 1, S.shape[0], 0, 0, base, eb, ldk, k0,
+                                        ptr(tmax), None) == 0
+        else:
+            assert L.omx_pack_f16_window(S.shape[0], k, ptr(sd), 1, S.shape[0], 0, base, None, None, ldk, k0,
+                                         None) == 0
+        return sd
+
+    keep = [pack(X1, PX, ex, 0, 0), pack(X2, PX, ex, shift, k), pack(Y1, PY, ey, 0, 0), pack(Y2, PY, ey, shift, k)]
+    C0 = rng.uniform(-1, 1, (n, m)).astype(np.float32)
+    Cd = dev(C0.ravel())
+    if fp8:
+        rc = L.omx_gemm_f8_packed_window(m, n, ldk, -1.0, ptr(PX), ptr(ex), shift, ldk, 0, ptr(PY), ptr(ey), shift,
+                                         ldk, 0, 1.0, ptr(Cd), m, None)
+    else:
+        rc = L.omx_gemm_f16_packed_window(m, n, ldk, -1.0, ptr(PX), None, shift, ldk, 0, ptr(PY), None, shift,
+                                          ldk, 0, 1.0, ptr(Cd), m, None)
+    assert rc == 0
+    got = host(Cd).reshape(n, m).T.astype(np.float64)
+    if fp8:
+        def dq(S):
+            out = np.zeros_like(S, dtype=np.float64)
+            for s0 in range(0, S.shape[0], 256):
+                for t in range(0, k, 256):
+                    blk = S[s0:s0 + 256, t:t + 256]
+                    e = orc.pow2_scale_exp(float(np.abs(blk).max()))
+                    out[s0:s0 + 256, t:t + 256] = orc.e4m3_bits_to_f32(orc.f32_to_e4m3_bits(np.ldexp(blk, e))) * 2.0 ** -e
+            return out
+        Xa, Ya, Xb, Yb = dq(X1)[shift:], dq(Y1)[shift:], dq(X2), dq(Y2)
+    else:
+        h = lambda S: S.astype(np.float16).astype(np.float64)
+        Xa, Ya, Xb, Yb = h(X1)[shift:], h(Y1)[shift:], h(X2), h(Y2)
+    exact = C0.T.astype(np.float64) - Xa @ Ya.T - Xb @ Yb.T
+    bound = (ldk + 2) * 2.0**-24 * (np.abs(Xa) @ np.abs(Ya).T + np.abs(Xb) @ np.abs(Yb).T) + 2.0**-23 * np.abs(exact)
+    assert np.all(np.abs(got - exact) <= bound)
+    # second window alone == the panel packed by itself
+    C1, C2 = dev(C0.ravel()), dev(C0.ravel())
+    if fp8:
+        P2 = empty(L.omx_panel_bytes(m, k, 1), torch.uint8)
+        Q2 = empty(L.omx_panel_bytes(n, k, 1), torch.uint8)
+        e2 = empty(L.omx_panel_exps_count(m, k), torch.int32)
+        f2 = empty(L.omx_panel_exps_count(n, k), torch.int32)
+        t2 = empty(L.omx_panel_exps_count(max(m, n), k), torch.float32)
+        assert L.omx_pack_f8(m, k, ptr(keep[1]), 1, m, ptr(P2), ptr(e2), ptr(t2), None) == 0
+        assert L.omx_pack_f8(n, k, ptr(keep[3]), 1, n, ptr(Q2), ptr(f2), ptr(t2), None) == 0
+        assert L.omx_gemm_f8_packed(m, n, k, -1.0, ptr(P2), ptr(e2), 0, ptr(Q2), ptr(f2), 0, 1.0, ptr(C1), m,
+                                    None) == 0
+        assert L.omx_gemm_f8_packed_window(m, n, k, -1.0, ptr(PX), ptr(ex), shift, ldk, k, ptr(PY), ptr(ey), shift,
+                                           ldk, k, 1.0, ptr(C2), m, None) == 0
+    else:
+        P2 = dev(pack16(X2.astype(np.float16)).view(np.int16))
+        Q2 = dev(pack16(Y2.astype(np.float16)).view(np.int16))
+        assert L.omx_gemm_f16_packed(m, n, k, -1.0, ptr(P2), 0, ptr(Q2), 0, 1.0, ptr(C1), m, None) == 0
+        assert L.omx_gemm_f16_packed_window(m, n, k, -1.0, ptr(PX), None, shift, ldk, k, ptr(PY), None, shift,
+                                            ldk, k, 1.0, ptr(C2), m, None) == 0
+    assert np.array_equal(host(C1), host(C2))
