@@ -201,62 +201,82 @@ __device__ __forceinline__ double block_rows_dot(const float *__restrict__ A, in
     return t;
 }
 
+// Lane's row `row` of the diagonal block at d0 (order dim), the 64 columns
+// from c0: row and column indices are clamped into the block, so every load
+// is unconditional and in bounds (all 64 can be in flight at once); entries
+// outside the block are masked where they are used.
+__device__ __forceinline__ void load_row64(const float *__restrict__ A, int64_t lda, int64_t d0, int dim,
+                                           int row, int c0, float (&a)[kSB]) {
+    const float *p = A + d0 + min(row, dim - 1) + d0 * lda;
+#pragma unroll
+    for (int c = 0; c < kSB; ++c) a[c] = p[(int64_t)min(c0 + c, dim - 1) * lda];
+}
+
 // x_D <- L_DD^-1 r_D for the unit-lower diagonal block at d0 of order dim
-// (<= 256); r_D in xs (LDS).  Sub-block sb: wave sb runs the 64-step chain
-// (lane l holds row sb*64 + l; x_c read from lane c), then the waves of
-// the later sub-blocks subtract M(rows, sub-block) x_sb.
+// (<= 256); r_D in xs (LDS).  Wave w owns rows [64 w, 64 w + 64) (lane l:
+// row 64 w + l) and holds its diagonal sub-block's rows in registers from
+// the start.  Sub-block sb: wave sb runs the 64-step chain (x_c read from
+// lane c), then every later wave subtracts its rows' product with x_sb,
+// using the off-diagonal sub-block it prefetched during the previous stage.
 __device__ __forceinline__ void diag_solve_lower(const float *__restrict__ A, int64_t lda, int64_t d0, int dim,
                                                  double *xs) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int sb = 0; sb * kSB < dim; ++sb) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nsb = (dim + kSB - 1) / kSB;
+    const int i = wave * kSB + lane;
+    float a[kSB], b[kSB];
+    if (wave < nsb) {
+        load_row64(A, lda, d0, dim, i, wave * kSB, a);
+        if (wave > 0) load_row64(A, lda, d0, dim, i, 0, b);
+    }
+#pragma unroll
+    for (int sb = 0; sb < kTB / kSB; ++sb) {
+        if (sb >= nsb) break;
         const int c0 = sb * kSB;
         if (wave == sb) {
-            const int i = c0 + lane;
-            float a[kSB];
-            const float *p = A + d0 + min(i, dim - 1);
-#pragma unroll
-            for (int c = 0; c < kSB; ++c)  // row and column clamped into the block
-                a[c] = (lane > c && i < dim) ? p[(d0 + min(c0 + c, dim - 1)) * lda] : 0.0f;
             double v = xs[i];
-            // a[c] is zero for lanes <= c, so the update needs no lane mask
 #pragma unroll
-            for (int c = 0; c < kSB; ++c) v = __builtin_fma(-bcast_lane(v, c), (double)a[c], v);
-            xs[i] = v;
+            for (int c = 0; c < kSB; ++c) {
+                const double m = (lane > c && i < dim) ? (double)a[c] : 0.0;
+                v = __builtin_fma(-bcast_lane(v, c), m, v);
+            }
+            if (i < dim) xs[i] = v;
         }
         __syncthreads();
-        if (wave > sb) {
-            const int i = wave * kSB + lane;
-            if (i < dim) {
-                const float *p = A + d0 + i + (d0 + c0) * lda;
-                double s = 0.0;
-                const int ce = min(kSB, dim - c0);
-#pragma unroll 16
-                for (int c = 0; c < ce; ++c) s = __builtin_fma((double)p[(int64_t)c * lda], xs[c0 + c], s);
-                xs[i] -= s;
-            }
+        if (wave > sb && wave < nsb) {
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < kSB; ++c) s = __builtin_fma(c0 + c < dim ? (double)b[c] : 0.0, xs[c0 + c], s);
+            if (i < dim) xs[i] -= s;
+            if (sb + 1 < wave) load_row64(A, lda, d0, dim, i, c0 + kSB, b);  // next stage's sub-block
         }
         __syncthreads();
     }
 }
 
 // x_D <- U_DD^-1 r_D for the upper non-unit diagonal block (sub-blocks from
-// the last to the first).  The quotient r / u is formed as q0 = r * y,
-// q = q0 + (r - u q0) y with y = 1/u: the correctly rounded quotient
-// (Markstein's correction with a correctly rounded reciprocal).
+// the last to the first; registers and prefetch as in the lower solve).  The
+// quotient r / u is formed as q0 = r * y, q = q0 + (r - u q0) y with
+// y = 1/u: the correctly rounded quotient (Markstein's correction with a
+// correctly rounded reciprocal).
 __device__ __forceinline__ void diag_solve_upper(const float *__restrict__ A, int64_t lda, int64_t d0, int dim,
                                                  double *xs) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nsb = (dim + kSB - 1) / kSB;
-    for (int sb = nsb - 1; sb >= 0; --sb) {
+    const int i = wave * kSB + lane;
+    float a[kSB], b[kSB];
+    float ud = 1.0f;
+    if (wave < nsb) {
+        load_row64(A, lda, d0, dim, i, wave * kSB, a);
+        ud = A[d0 + min(i, dim - 1) + (d0 + min(i, dim - 1)) * lda];
+        if (wave < nsb - 1) load_row64(A, lda, d0, dim, i, (nsb - 1) * kSB, b);
+    }
+#pragma unroll
+    for (int t = 0; t < kTB / kSB; ++t) {
+        const int sb = nsb - 1 - t;
+        if (sb < 0) break;
         const int c0 = sb * kSB;
         if (wave == sb) {
-            const int i = c0 + lane;
-            float a[kSB];  // strictly upper part of the lane's row
-            const float *p = A + d0 + min(i, dim - 1);
-#pragma unroll
-            for (int c = 0; c < kSB; ++c)  // row and column clamped into the block
-                a[c] = (lane < c && c0 + c < dim) ? p[(d0 + min(c0 + c, dim - 1)) * lda] : 0.0f;
-            const double u = (i < dim) ? (double)A[d0 + i + (d0 + i) * lda] : 1.0;
+            const double u = (i < dim) ? (double)ud : 1.0;
             const double y = 1.0 / u;
             auto quot = [&](double r) {
                 const double q0 = r * y;
@@ -267,18 +287,19 @@ __device__ __forceinline__ void diag_solve_upper(const float *__restrict__ A, in
             // then on, so its final quotient below equals x_c.
             double v = (i < dim) ? xs[i] : 0.0;
 #pragma unroll
-            for (int c = kSB - 1; c >= 0; --c) v = __builtin_fma(-bcast_lane(quot(v), c), (double)a[c], v);
+            for (int c = kSB - 1; c >= 0; --c) {
+                const double m = (lane < c && c0 + c < dim) ? (double)a[c] : 0.0;
+                v = __builtin_fma(-bcast_lane(quot(v), c), m, v);
+            }
             if (i < dim) xs[i] = quot(v);
         }
         __syncthreads();
         if (wave < sb) {
-            const int i = wave * kSB + lane;
-            const float *p = A + d0 + i + (d0 + c0) * lda;
             double s = 0.0;
-            const int ce = min(kSB, dim - c0);
-#pragma unroll 16
-            for (int c = 0; c < ce; ++c) s = __builtin_fma((double)p[(int64_t)c * lda], xs[c0 + c], s);
+#pragma unroll
+            for (int c = 0; c < kSB; ++c) s = __builtin_fma(c0 + c < dim ? (double)b[c] : 0.0, xs[c0 + c], s);
             xs[i] -= s;
+            if (sb - 1 > wave) load_row64(A, lda, d0, dim, i, c0 - kSB, b);  // next stage's sub-block
         }
         __syncthreads();
     }
