@@ -61,7 +61,7 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (the reference kernel for PMC byte counters)")
     ap.add_argument("--gemm-sweep", action="store_true",
-                    help="run only the FP16 trailing-update GEMMs of one factorisation (shapes m = n-(k+1)b) "
+                    help="run only the FP16 paired trailing-update GEMMs of one factorisation (m x m x 2b, m = n-(2p+4)b) "
                          "on the default stream, for counter collection")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the Schur GEMM traffic from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
@@ -241,49 +241,64 @@ def summarize_pmc(fetch_csv, write_csv, n, b):
     nl = len(res["FETCH_SIZE"][1])
     fetch = sum(res["FETCH_SIZE"][1])
     write = sum(res["WRITE_SIZE"][1])
-    # algorithmic bytes of the trailing updates: FP32 C read + write and the
-    # two packed FP16 panels, per launch m = n - (k+1) b, k = 1 .. nb-2
-    nb = n // b
-    ms = [n - (k + 1) * b for k in range(1, nb) if n - (k + 1) * b > 0]
-    alg = sum(8.0 * m * m + 2 * 2.0 * m * b for m in ms)
+    # algorithmic bytes of the paired trailing updates: FP32 C read + write
+    # and the two packed FP16 panels of K = 2b columns, m = n - (2p+4) b
+    ms = bulk_shapes(n, b)
+    alg = sum(8.0 * m * m + 2 * 2.0 * m * 2 * b for m in ms)
     print(f"kernel: k_gemm_f16<false,4> (trailing update) at n={n} b={b}")
     print(f"launches: {nl} (algorithmic model: {len(ms)})")
     print(f"fetch_calibration_factor: {res['FETCH_SIZE'][0]:.4f}  (bytes per counted byte, 1 GiB copy)")
     print(f"write_calibration_factor: {res['WRITE_SIZE'][0]:.4f}")
     print(f"fetch_bytes_total: {fetch:.4e}")
     print(f"write_bytes_total: {write:.4e}")
-    print(f"algorithmic_bytes_total: {alg:.4e}  (8 m^2 for C + 4 m b for the panels)")
+    print(f"algorithmic_bytes_total: {alg:.4e}  (8 m^2 for C + 8 m b for the two K = 2b panels)")
     print(f"traffic_over_algorithmic: {(fetch + write) / alg:.3f}")
     print(f"traffic_bytes_per_launch: {(fetch + write) / max(nl, 1):.6e}")
 
 
+def bulk_shapes(n, b):
+    """Row counts m of the paired trailing updates of one factorisation:
+    panels 2p and 2p+1 update A[f:, f:] with f = (2p + 4) b, K = 2 b."""
+    nb = n // b
+    out = []
+    p = 0
+    while 2 * p + 2 < nb:
+        m = n - (2 * p + 4) * b
+        if m > 0:
+            out.append(m)
+        p += 1
+    return out
+
+
 def gemm_sweep(n, b, seed):
-    """The 78 FP16 trailing updates of one factorisation, one launch each,
-    on the null stream, with packed panels of the right size and the
-    generated matrix as C; prints the event-timed rate."""
+    """The FP16 paired trailing updates of one factorisation (m x m x 2b,
+    m = n - (2p+4) b), one launch each, on the null stream, with packed
+    panels of the right size and the generated matrix as C; prints the
+    event-timed rate."""
     import torch
     from openmxp_amd import _lib
     L = _lib.lib()
     A = torch.empty(n * n, dtype=torch.float32, device="cuda")
     d = torch.empty(n, dtype=torch.float64, device="cuda")
     assert L.omx_matgen_f32(n, seed, A.data_ptr(), n, d.data_ptr(), None) == 0
-    pb = L.omx_panel_bytes(n - b, b, 0)
+    pb = L.omx_panel_bytes(n - 2 * b, 2 * b, 0)
     X = (torch.rand(pb // 2, device="cuda") - 0.5).half()
     Y = (torch.rand(pb // 2, device="cuda") - 0.5).half()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     flops = 0.0
+    ms = bulk_shapes(n, b)
     e0.record()
-    for k in range(1, n // b - 1):
-        m = n - (k + 1) * b
-        off = (k + 1) * b * (n + 1)
-        assert L.omx_gemm_f16_packed(m, m, b, -1.0, X.data_ptr(), 0, Y.data_ptr(), 0, 1.0,
+    for m in ms:
+        f = n - m
+        off = f * (n + 1)
+        assert L.omx_gemm_f16_packed(m, m, 2 * b, -1.0, X.data_ptr(), 0, Y.data_ptr(), 0, 1.0,
                                      A.data_ptr() + 4 * off, n, None) == 0
-        flops += 2.0 * m * m * b
+        flops += 2.0 * m * m * 2 * b
     e1.record()
     torch.cuda.synchronize()
     t = e0.elapsed_time(e1) * 1e-3
-    log(f"gemm sweep n={n} b={b}: {n // b - 2} launches, {t * 1e3:.1f} ms, {flops / t * 1e-12:.1f} TFLOP/s")
+    log(f"gemm sweep n={n} b={b}: {len(ms)} launches, {t * 1e3:.1f} ms, {flops / t * 1e-12:.1f} TFLOP/s")
 
 
 def main():

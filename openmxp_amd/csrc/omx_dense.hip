@@ -9,6 +9,8 @@
 // multiplication by the reciprocal, as sgetrf_nopiv_naive does
 // (sgetrf_nopiv.cpp:27-61) and as reference BLAS strsm does for a non-unit
 // diagonal.
+#include <utility>
+
 #include "omx_common.hpp"
 
 namespace {
@@ -75,77 +77,168 @@ k_sgemm_nn(int64_t M, int64_t N, int64_t K, float alpha, const float *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// 64 x 64 diagonal piece of the LU: unblocked right-looking LU in LDS.
+// 64 x 64 diagonal piece of the LU: unblocked right-looking LU in one wave.
+// Lane i holds row i in registers.  Step k: lane k publishes its row
+// through LDS, every lane below scales its column-k entry by 1/u_kk (the
+// multiplier) and subtracts multiplier x row k from the rest of its row.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kNB)
 k_getrf_diag64(float *__restrict__ A, int64_t lda, int64_t row0, int *__restrict__ info) {
-    __shared__ float a[kNB][kNB + 1];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < kNB * kNB; e += 256) a[e & 63][e >> 6] = A[(e & 63) + (int64_t)(e >> 6) * lda];
-    __syncthreads();
+    __shared__ float urow[kNB];
+    const int i = threadIdx.x;
+    float r[kNB];
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) r[j] = A[i + (int64_t)j * lda];
+#pragma unroll
     for (int k = 0; k < kNB; ++k) {
-        const float akk = a[k][k];
-        if (tid == 0 && akk == 0.0f && info) atomicMin(info, (int)(row0 + k + 1));
-        const float inv = 1.0f / akk;
-        if (tid > k && tid < kNB) a[tid][k] *= inv;
+        if (i == k) {
+#pragma unroll
+            for (int j = k; j < kNB; ++j) urow[j] = r[j];
+        }
         __syncthreads();
-        for (int e = tid; e < kNB * kNB; e += 256) {
-            const int i = e & 63, j = e >> 6;
-            if (i > k && j > k) a[i][j] -= a[i][k] * a[k][j];
+        const float ukk = urow[k];
+        if (i == 0 && ukk == 0.0f && info) atomicMin(info, (int)(row0 + k + 1));
+        const float inv = 1.0f / ukk;
+        if (i > k) {
+            const float l = r[k] * inv;
+            r[k] = l;
+#pragma unroll
+            for (int j = k + 1; j < kNB; ++j) r[j] = __builtin_fmaf(-l, urow[j], r[j]);
         }
         __syncthreads();
     }
-    for (int e = tid; e < kNB * kNB; e += 256) A[(e & 63) + (int64_t)(e >> 6) * lda] = a[e & 63][e >> 6];
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) A[i + (int64_t)j * lda] = r[j];
 }
 
-// Rows: X(i, :) <- X(i, :) * U^-1 for a 64 x 64 upper non-unit U (RUNN).
-// Column j of the row is b_j minus earlier columns, times 1/U_jj.
+// The 64 x 64 triangle T (column-major, leading dimension ldt) into LDS:
+// t[r][c] = T(r, c).
+__device__ __forceinline__ void load_tri64(const float *__restrict__ T, int64_t ldt, float (*t)[kNB + 1]) {
+    for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) t[e & 63][e >> 6] = T[(e & 63) + (int64_t)(e >> 6) * ldt];
+    __syncthreads();
+}
+
+// Solves of one row (RUNN) or one column (LLNU) of 64 entries by a quad of
+// lanes: lane q of the quad owns entries q, q + 4, ..., q + 60 (16 registers,
+// index s <-> entry q + 4 s).  Step j: the owner (lane j % 4) finalises
+// entry j, the quad reads it with a DPP broadcast, and every lane updates
+// its later entries.  Each entry still receives its terms in the order
+// j = 0, 1, ... with one fused multiply-add each, as in column-by-column
+// substitution, so the results are those of the one-thread sweep; the
+// dependent chain per step is one multiply (or FMA) and one broadcast.
+
+// rd[j] = 1 / t[j][j] (the diagonal's reciprocals, shared by the workgroup)
+__device__ __forceinline__ void diag_recip64(const float (*t)[kNB + 1], float *rd) {
+    if (threadIdx.x < kNB) rd[threadIdx.x] = 1.0f / t[threadIdx.x][threadIdx.x];
+    __syncthreads();
+}
+
+// value of v in quad lane o (o compile-time) for every lane of the quad
+template <int o>
+__device__ __forceinline__ float quad_bcast(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), o | (o << 2) | (o << 4) | (o << 6),
+                                                              0xF, 0xF, false));
+}
+
+template <int j>
+__device__ __forceinline__ void runn_step(float (&x)[16], int q, const float (*u)[kNB + 1], const float *rd) {
+    constexpr int o = j & 3, s = j >> 2;
+    const float cand = x[s] * rd[j];
+    const float xj = quad_bcast<o>(cand);
+    if (q == o) x[s] = xj;
+    if (q > o) x[s] = __builtin_fmaf(-xj, u[j][q + 4 * s], x[s]);
+#pragma unroll
+    for (int t = s + 1; t < 16; ++t) x[t] = __builtin_fmaf(-xj, u[j][q + 4 * t], x[t]);
+}
+
+template <int j>
+__device__ __forceinline__ void llnu_step(float (&y)[16], int q, const float (*l)[kNB + 1]) {
+    constexpr int o = j & 3, s = j >> 2;
+    const float t = quad_bcast<o>(y[s]);
+    if (q > o) y[s] = __builtin_fmaf(-t, l[q + 4 * s][j], y[s]);
+#pragma unroll
+    for (int w = s + 1; w < 16; ++w) y[w] = __builtin_fmaf(-t, l[q + 4 * w][j], y[w]);
+}
+
+template <int... J>
+__device__ __forceinline__ void runn_all(float (&x)[16], int q, const float (*u)[kNB + 1], const float *rd,
+                                         std::integer_sequence<int, J...>) {
+    (runn_step<J>(x, q, u, rd), ...);
+}
+
+template <int... J>
+__device__ __forceinline__ void llnu_all(float (&y)[16], int q, const float (*l)[kNB + 1],
+                                         std::integer_sequence<int, J...>) {
+    (llnu_step<J>(y, q, l), ...);
+}
+
+// Row i: X(i, :) <- X(i, :) * U^-1 for a 64 x 64 upper non-unit U (RUNN);
+// x_j = (b_j - sum_{l<j} x_l u_lj) * rd_j with rd_j = 1 / u_jj (correctly
+// rounded, formed once per workgroup).  Called by the four lanes of a quad
+// with the same i (q = lane % 4).
+__device__ __forceinline__ void row_runn64(int64_t i, int q, const float (*u)[kNB + 1], const float *rd,
+                                           float *__restrict__ X, int64_t ldx) {
+    float x[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) x[t] = X[i + (q + 4 * t) * ldx];
+    runn_all(x, q, u, rd, std::make_integer_sequence<int, kNB>{});
+#pragma unroll
+    for (int t = 0; t < 16; ++t) X[i + (q + 4 * t) * ldx] = x[t];
+}
+
+// Column j: Y(:, j) <- L^-1 Y(:, j) for a 64 x 64 unit lower L (LLNU);
+// y_i -= y_c l_ic for c < i, c ascending.  Called by a quad as above.
+__device__ __forceinline__ void col_llnu64(int64_t j, int q, const float (*l)[kNB + 1], float *__restrict__ Y,
+                                           int64_t ldy) {
+    float y[16];
+    float *col = Y + j * ldy;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) y[t] = col[q + 4 * t];
+    llnu_all(y, q, l, std::make_integer_sequence<int, kNB>{});
+#pragma unroll
+    for (int t = 0; t < 16; ++t) col[q + 4 * t] = y[t];
+}
+
+// 64 rows (or columns) per workgroup of 256 threads, a quad per row.  The
+// quad's lanes stay together: a row past the end exits as a whole quad.
 __global__ void __launch_bounds__(256)
 k_trsm_rows_runn64(int64_t m, const float *__restrict__ U, int64_t ldu, float *__restrict__ X,
                    int64_t ldx) {
     __shared__ float u[kNB][kNB + 1];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < kNB * kNB; e += 256) u[e & 63][e >> 6] = U[(e & 63) + (int64_t)(e >> 6) * ldu];
-    __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * 256 + tid;
-    if (i >= m) return;
-    float x[kNB];
-#pragma unroll
-    for (int j = 0; j < kNB; ++j) x[j] = X[i + j * ldx];
-#pragma unroll
-    for (int j = 0; j < kNB; ++j) {
-        float v = x[j];
-#pragma unroll
-        for (int l = 0; l < j; ++l) v -= x[l] * u[l][j];
-        x[j] = v * (1.0f / u[j][j]);
-    }
-#pragma unroll
-    for (int j = 0; j < kNB; ++j) X[i + j * ldx] = x[j];
+    __shared__ float rd[kNB];
+    load_tri64(U, ldu, u);
+    diag_recip64(u, rd);
+    const int64_t i = (int64_t)blockIdx.x * 64 + (threadIdx.x >> 2);
+    if (i < m) row_runn64(i, threadIdx.x & 3, u, rd, X, ldx);
 }
 
-// Columns: Y(:, j) <- L^-1 Y(:, j) for a 64 x 64 unit lower L (LLNU).
 __global__ void __launch_bounds__(256)
 k_trsm_cols_llnu64(int64_t n, const float *__restrict__ L, int64_t ldl, float *__restrict__ Y,
                    int64_t ldy) {
     __shared__ float l[kNB][kNB + 1];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < kNB * kNB; e += 256) l[e & 63][e >> 6] = L[(e & 63) + (int64_t)(e >> 6) * ldl];
-    __syncthreads();
-    const int64_t j = (int64_t)blockIdx.x * 256 + tid;
-    if (j >= n) return;
-    float y[kNB];
-    const float *col = Y + j * ldy;
-#pragma unroll
-    for (int i = 0; i < kNB; ++i) y[i] = col[i];
-#pragma unroll
-    for (int c = 0; c < kNB; ++c) {
-        const float t = y[c];
-#pragma unroll
-        for (int i = c + 1; i < kNB; ++i) y[i] -= t * l[i][c];
+    load_tri64(L, ldl, l);
+    const int64_t j = (int64_t)blockIdx.x * 64 + (threadIdx.x >> 2);
+    if (j < n) col_llnu64(j, threadIdx.x & 3, l, Y, ldy);
+}
+
+// Both solves of one 64-column step of the blocked LU in one launch, with
+// the factored diagonal block D at Ajj: workgroups [0, nrw) take the rows
+// below D (RUNN with D's U), the rest the columns right of D (LLNU with D's
+// unit L).  rest = rows below = columns right of D; 64 per workgroup.
+__global__ void __launch_bounds__(256)
+k_getrf_trsm64(int64_t rest, int nrw, float *__restrict__ Ajj, int64_t lda) {
+    __shared__ float d[kNB][kNB + 1];
+    __shared__ float rd[kNB];
+    load_tri64(Ajj, lda, d);
+    diag_recip64(d, rd);
+    const int q = threadIdx.x & 3;
+    if ((int)blockIdx.x < nrw) {
+        const int64_t i = (int64_t)blockIdx.x * 64 + (threadIdx.x >> 2);
+        if (i < rest) row_runn64(i, q, d, rd, Ajj + kNB, lda);
+    } else {
+        const int64_t j = (int64_t)(blockIdx.x - nrw) * 64 + (threadIdx.x >> 2);
+        if (j < rest) col_llnu64(j, q, d, Ajj + kNB * lda, lda);
     }
-    float *out = Y + j * ldy;
-#pragma unroll
-    for (int i = 0; i < kNB; ++i) out[i] = y[i];
 }
 
 // Rows: X(i, :) <- X(i, :) * L^-1 for a 64 x 64 unit lower L (RLNU),
@@ -154,10 +247,8 @@ __global__ void __launch_bounds__(256)
 k_trsm_rows_rlnu64(int64_t m, const float *__restrict__ L, int64_t ldl, float *__restrict__ X,
                    int64_t ldx) {
     __shared__ float l[kNB][kNB + 1];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < kNB * kNB; e += 256) l[e & 63][e >> 6] = L[(e & 63) + (int64_t)(e >> 6) * ldl];
-    __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * 256 + tid;
+    load_tri64(L, ldl, l);
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= m) return;
     float x[kNB];
 #pragma unroll
@@ -230,15 +321,12 @@ int omx_getrf_nopiv_f32(int64_t n, float *A, int64_t lda, int *info, void *strea
     }
     for (int64_t j = 0; j < n; j += kNB) {
         float *Ajj = A + j + j * lda;
-        hipLaunchKernelGGL(k_getrf_diag64, dim3(1), dim3(256), 0, st, Ajj, lda, j, info);
+        hipLaunchKernelGGL(k_getrf_diag64, dim3(1), dim3(kNB), 0, st, Ajj, lda, j, info);
         OMX_LAUNCH_CHECK();
         const int64_t rest = n - j - kNB;
         if (rest <= 0) break;
-        hipLaunchKernelGGL(k_trsm_rows_runn64, dim3((unsigned)omx_cdiv(rest, 256)), dim3(256), 0, st, rest,
-                           Ajj, lda, Ajj + kNB, lda);
-        OMX_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_trsm_cols_llnu64, dim3((unsigned)omx_cdiv(rest, 256)), dim3(256), 0, st, rest,
-                           Ajj, lda, Ajj + kNB * lda, lda);
+        const int nrw = (int)omx_cdiv(rest, 64);
+        hipLaunchKernelGGL(k_getrf_trsm64, dim3((unsigned)(2 * nrw)), dim3(256), 0, st, rest, nrw, Ajj, lda);
         OMX_LAUNCH_CHECK();
         int rc = sgemm(rest, rest, kNB, -1.0f, Ajj + kNB, lda, Ajj + kNB * lda, lda, 1.0f,
                        Ajj + kNB + kNB * lda, lda, st);
@@ -257,8 +345,9 @@ int omx_trsm_f32(int variant, int64_t m, int64_t n, const float *T, int64_t ldt,
         if (n % kNB || ldt < n) return OMX_ERR_ARG;
         const unsigned g = (unsigned)omx_cdiv(m, 256);
         if (variant == OMX_TRSM_RUNN) {
+            const unsigned g4 = (unsigned)omx_cdiv(m, 64);  // a lane quad per row
             for (int64_t j = 0; j < n; j += kNB) {
-                hipLaunchKernelGGL(k_trsm_rows_runn64, dim3(g), dim3(256), 0, st, m, T + j + j * ldt, ldt,
+                hipLaunchKernelGGL(k_trsm_rows_runn64, dim3(g4), dim3(256), 0, st, m, T + j + j * ldt, ldt,
                                    B + j * ldb, ldb);
                 OMX_LAUNCH_CHECK();
                 const int64_t rest = n - j - kNB;
@@ -284,11 +373,11 @@ int omx_trsm_f32(int variant, int64_t m, int64_t n, const float *T, int64_t ldt,
     if (variant == OMX_TRSM_LLNU) {
         // inverse of an m x m unit lower triangle times B (m x n), m % 64 == 0
         if (m % kNB || ldt < m) return OMX_ERR_ARG;
-        const int64_t nmax = (int64_t)65535 * 256;
+        const int64_t nmax = (int64_t)65535 * 64;  // a lane quad per column
         for (int64_t i = 0; i < m; i += kNB) {
             for (int64_t c = 0; c < n; c += nmax) {
                 const int64_t nn = (n - c < nmax) ? n - c : nmax;
-                hipLaunchKernelGGL(k_trsm_cols_llnu64, dim3((unsigned)omx_cdiv(nn, 256)), dim3(256), 0, st, nn,
+                hipLaunchKernelGGL(k_trsm_cols_llnu64, dim3((unsigned)omx_cdiv(nn, 64)), dim3(256), 0, st, nn,
                                    T + i + i * ldt, ldt, B + i + c * ldb, ldb);
                 OMX_LAUNCH_CHECK();
             }
