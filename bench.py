@@ -17,10 +17,13 @@ Two methods are measured on the same system:
            per LU step, GMRES-IR; if the refinement fails the system is
            generated again and factored in FP16, inside the timed region.
 `value` is the learned method's rate; the line carries the plain rate and
-the ratio.  It also carries the Schur GEMM roofline of the plain run (the
-trailing-update launches, timed with HIP events on the stream they run on),
-the forward errors against an FP64 refinement to the floor, and the CPU
-oracle timed on this host as the baseline.
+the ratio.  `roofline` is that of the headline run's dominant kernel, the
+FP8 trailing-update GEMM of the learned run (`roofline_plain_fp16`: the FP16
+one of the plain run), each timed with HIP events on the stream its launches
+run on, with the HBM bytes per launch from the committed rocprofv3 counter
+summary.  The line also carries the forward errors against an FP64
+refinement to the floor and the CPU oracle timed on this host as the
+baseline.
 """
 from __future__ import annotations
 
@@ -37,7 +40,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP16_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
 PEAK_FP8_TFLOPS = 5000.0   # MI355X dense FP8 MFMA
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "gemm_f16_traffic_n204800.txt")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "gemm_traffic_n204800.txt")
 
 
 def hpl_flops(n: int) -> float:
@@ -204,20 +207,22 @@ def cpu_baseline(n, b, alt, seed):
                 iters=res["iters"], info=info)
 
 
-def pmc_traffic():
-    """HBM bytes per launch of the FP16 Schur GEMM from the committed PMC
-    summary (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated), if any."""
+def pmc_traffic(tag):
+    """HBM bytes per launch of the FP16 ("f16") or FP8 ("f8") trailing-update
+    GEMM from the committed PMC summary (rocprofv3 FETCH_SIZE / WRITE_SIZE,
+    copy-calibrated), if any."""
     try:
         txt = open(PMC_SUMMARY).read()
     except OSError:
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    m = re.search(tag + r"\.traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
     return float(m.group(1)) if m else None
 
 
 def summarize_pmc(fetch_csv, write_csv, n, b):
-    """Schur GEMM HBM traffic per launch from two rocprofv3 PMC runs, each
-    counter scaled by what it reads for the 1 GiB copy of the same run."""
+    """Schur GEMM HBM traffic per launch (FP16 and FP8 trailing updates) from
+    two rocprofv3 PMC runs of the GEMM sweep, each counter scaled by what it
+    reads for the 1 GiB copy of the same run."""
     import csv
 
     def rows(path, counter):
@@ -228,32 +233,36 @@ def summarize_pmc(fetch_csv, write_csv, n, b):
         return out
 
     copy_bytes = float(1 << 30)
-    res = {}
+    cal = {}
+    data = {}
     for path, counter in ((fetch_csv, "FETCH_SIZE"), (write_csv, "WRITE_SIZE")):
         rs = rows(path, counter)
-        cal = [v for k, v in rs if "copyBuffer" in k]
-        if not cal:
+        c = [v for k, v in rs if "copyBuffer" in k]
+        if not c:
             raise SystemExit(f"{path}: no copy kernel to calibrate {counter}")
         # the copy reads and writes 1 GiB each
-        factor = copy_bytes / cal[0]
-        g = [v * factor for k, v in rs if "k_gemm_f16<false" in k]
-        res[counter] = (factor, g)
-    nl = len(res["FETCH_SIZE"][1])
-    fetch = sum(res["FETCH_SIZE"][1])
-    write = sum(res["WRITE_SIZE"][1])
-    # algorithmic bytes of the paired trailing updates: FP32 C read + write
-    # and the two packed FP16 panels of K = 2b columns, m = n - (2p+4) b
+        cal[counter] = copy_bytes / c[0]
+        data[counter] = rs
     ms = bulk_shapes(n, b)
-    alg = sum(8.0 * m * m + 2 * 2.0 * m * 2 * b for m in ms)
-    print(f"kernel: k_gemm_f16<false,4> (trailing update) at n={n} b={b}")
-    print(f"launches: {nl} (algorithmic model: {len(ms)})")
-    print(f"fetch_calibration_factor: {res['FETCH_SIZE'][0]:.4f}  (bytes per counted byte, 1 GiB copy)")
-    print(f"write_calibration_factor: {res['WRITE_SIZE'][0]:.4f}")
-    print(f"fetch_bytes_total: {fetch:.4e}")
-    print(f"write_bytes_total: {write:.4e}")
-    print(f"algorithmic_bytes_total: {alg:.4e}  (8 m^2 for C + 8 m b for the two K = 2b panels)")
-    print(f"traffic_over_algorithmic: {(fetch + write) / alg:.3f}")
-    print(f"traffic_bytes_per_launch: {(fetch + write) / max(nl, 1):.6e}")
+    print(f"workload: the paired trailing updates of one factorisation, n={n} b={b} (m x m x 2b, "
+          f"m = n - (2p+4) b, {len(ms)} launches per precision)")
+    print(f"fetch_calibration_factor: {cal['FETCH_SIZE']:.4f}  (bytes per counted byte, 1 GiB copy)")
+    print(f"write_calibration_factor: {cal['WRITE_SIZE']:.4f}")
+    for tag, kname, eb in (("f16", "k_gemm_f16<false", 2.0), ("f8", "k_gemm_f8<false", 1.0)):
+        fetch = [v * cal["FETCH_SIZE"] for k, v in data["FETCH_SIZE"] if kname in k]
+        write = [v * cal["WRITE_SIZE"] for k, v in data["WRITE_SIZE"] if kname in k]
+        if not fetch or not write:
+            continue
+        nl = len(fetch)
+        # algorithmic bytes: FP32 C read + write, the two packed panels of K = 2b
+        alg = sum(8.0 * m * m + 2 * eb * m * 2 * b for m in ms)
+        print(f"{tag}.kernel: {kname},4> (trailing update)")
+        print(f"{tag}.launches: {nl} (algorithmic model: {len(ms)})")
+        print(f"{tag}.fetch_bytes_total: {sum(fetch):.4e}")
+        print(f"{tag}.write_bytes_total: {sum(write):.4e}")
+        print(f"{tag}.algorithmic_bytes_total: {alg:.4e}  (8 m^2 for C + {2 * eb:.0f} m 2b for the two panels)")
+        print(f"{tag}.traffic_over_algorithmic: {(sum(fetch) + sum(write)) / alg:.3f}")
+        print(f"{tag}.traffic_bytes_per_launch: {(sum(fetch) + sum(write)) / max(nl, 1):.6e}")
 
 
 def bulk_shapes(n, b):
@@ -270,35 +279,48 @@ def bulk_shapes(n, b):
     return out
 
 
-def gemm_sweep(n, b, seed):
-    """The FP16 paired trailing updates of one factorisation (m x m x 2b,
-    m = n - (2p+4) b), one launch each, on the null stream, with packed
-    panels of the right size and the generated matrix as C; prints the
-    event-timed rate."""
+def gemm_sweep(n, b, seed, precs=("fp16", "fp8")):
+    """The paired trailing updates of one factorisation (m x m x 2b,
+    m = n - (2p+4) b), one launch each, on the null stream, first with FP16
+    then with FP8 packed panels of the right size (random contents) and the
+    generated matrix as C; prints the event-timed rates."""
     import torch
     from openmxp_amd import _lib
     L = _lib.lib()
     A = torch.empty(n * n, dtype=torch.float32, device="cuda")
     d = torch.empty(n, dtype=torch.float64, device="cuda")
     assert L.omx_matgen_f32(n, seed, A.data_ptr(), n, d.data_ptr(), None) == 0
-    pb = L.omx_panel_bytes(n - 2 * b, 2 * b, 0)
-    X = (torch.rand(pb // 2, device="cuda") - 0.5).half()
-    Y = (torch.rand(pb // 2, device="cuda") - 0.5).half()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    flops = 0.0
     ms = bulk_shapes(n, b)
-    e0.record()
-    for m in ms:
-        f = n - m
-        off = f * (n + 1)
-        assert L.omx_gemm_f16_packed(m, m, 2 * b, -1.0, X.data_ptr(), 0, Y.data_ptr(), 0, 1.0,
-                                     A.data_ptr() + 4 * off, n, None) == 0
-        flops += 2.0 * m * m * 2 * b
-    e1.record()
-    torch.cuda.synchronize()
-    t = e0.elapsed_time(e1) * 1e-3
-    log(f"gemm sweep n={n} b={b}: {len(ms)} launches, {t * 1e3:.1f} ms, {flops / t * 1e-12:.1f} TFLOP/s")
+    for prec in precs:
+        fp8 = prec == "fp8"
+        pb = L.omx_panel_bytes(n - 2 * b, 2 * b, 1 if fp8 else 0)
+        if fp8:
+            # e4m3 codes below 0x78 (no NaN), unit scales
+            X = torch.randint(0, 0x78, (pb,), dtype=torch.uint8, device="cuda")
+            Y = torch.randint(0, 0x78, (pb,), dtype=torch.uint8, device="cuda")
+            ex = torch.zeros(L.omx_panel_exps_count(n - 2 * b, 2 * b), dtype=torch.int32, device="cuda")
+        else:
+            X = (torch.rand(pb // 2, device="cuda") - 0.5).half()
+            Y = (torch.rand(pb // 2, device="cuda") - 0.5).half()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        flops = 0.0
+        e0.record()
+        for m in ms:
+            f = n - m
+            Cp = A.data_ptr() + 4 * f * (n + 1)
+            if fp8:
+                rc = L.omx_gemm_f8_packed(m, m, 2 * b, -1.0, X.data_ptr(), ex.data_ptr(), 0, Y.data_ptr(),
+                                          ex.data_ptr(), 0, 1.0, Cp, n, None)
+            else:
+                rc = L.omx_gemm_f16_packed(m, m, 2 * b, -1.0, X.data_ptr(), 0, Y.data_ptr(), 0, 1.0, Cp, n, None)
+            assert rc == 0
+            flops += 2.0 * m * m * 2 * b
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) * 1e-3
+        log(f"gemm sweep {prec} n={n} b={b}: {len(ms)} launches, {t * 1e3:.1f} ms, {flops / t * 1e-12:.1f} TFLOP/s")
+        del X, Y
 
 
 def main():
@@ -362,6 +384,10 @@ def main():
         log(f"CPU baseline: oracle solve at n={args.cpu_n} b={args.cpu_b}")
         cpu = cpu_baseline(args.cpu_n, args.cpu_b, args.alt, args.seed)
     head = aL if aL else a16
+    roof16 = {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
+              "frac": round(achieved / PEAK_FP16_TFLOPS, 4), "traffic": pmc_traffic("f16"),
+              "kernel": "k_gemm_f16 paired trailing update (alpha=-1, beta=1), plain run",
+              "launches": a16["gemm_launches"]}
     line = {
         "metric": "LU+IR TFLOP/s @N=204800 1 GPU, learned FP8/FP16 vs plain FP16; resid<16",
         "value": round(head["tflops"], 3),
@@ -387,16 +413,21 @@ def main():
         "forward_error": fwdL if aL else fwd16,
         "lu_s": round(head["lu_s"], 4),
         "ir_s": round(head["ir_s"], 4),
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_FP16_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP16_TFLOPS, 4), "traffic": pmc_traffic(),
-                     "kernel": "k_gemm_f16 trailing update (alpha=-1, beta=1), plain run",
-                     "launches": a16["gemm_launches"]},
+        "roofline": roof16,
+        "roofline_plain_fp16": roof16,
         "plain_fp16": {"tflops": round(a16["tflops"], 3), "ms_per_step": round(a16["ms"], 2), "valid": a16["ok"],
                        "harness": a16["harness"], "ir_steps": a16["ir_steps"], "forward_error": fwd16,
                        "lu_s": round(a16["lu_s"], 4), "ir_s": round(a16["ir_s"], 4)},
     }
     if aL:
         achL = aL["gemm_flops"] / (aL["gemm_ms"] * 1e-3) * 1e-12
+        # the headline run's dominant kernel: its trailing updates (FP8 where
+        # the schedule chose FP8; all of them when fp8_steps == steps)
+        line["roofline"] = {"bound": "mfma", "achieved": round(achL, 1), "peak": PEAK_FP8_TFLOPS, "unit": "TFLOP/s",
+                            "frac": round(achL / PEAK_FP8_TFLOPS, 4), "traffic": pmc_traffic("f8"),
+                            "kernel": "k_gemm_f8 paired trailing update (alpha=-1, beta=1), learned run",
+                            "launches": aL["gemm_launches"],
+                            "fp8_steps": aL["fp8_steps"], "steps": aL["steps_total"]}
         line["learned"] = {"speedup_vs_plain_fp16": round(aL["tflops"] / a16["tflops"], 3),
                            "fp8_steps": aL["fp8_steps"], "steps": aL["steps_total"], "inner_iters": aL["inner"],
                            "fell_back_to_fp16": aL["fell_back"], "trailing_gemm_tflops": round(achL, 1),

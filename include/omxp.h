@@ -336,6 +336,17 @@ int omx_getrf_nopiv_f32(int64_t n, float *A, int64_t lda, int *info,
 int omx_trsm_f32(int variant, int64_t m, int64_t n, const float *T,
                  int64_t ldt, float *B, int64_t ldb, void *stream);
 
+/* X = T^-1 as the triangular solve of the identity, X <- I * T^-1, for
+ * variant OMX_TRSM_RUNN (T upper, non-unit) or OMX_TRSM_RLNU (T lower,
+ * unit); n is a multiple of 64 and at most 65535.  Only the rows of each
+ * 64-column block that can be non-zero are solved (the inverse of a
+ * triangle is a triangle): those entries equal omx_identity_f32 followed by
+ * omx_trsm_f32 bit for bit, the rest are the identity's zeros.  Replaces
+ * gen_identity_mat + GPUBLAS_STRSM on the identity in the -alt 1/2 panel
+ * updates (panel_trf_gdirect.hpp:87-92, :120-125). */
+int omx_trsm_identity_f32(int variant, int64_t n, const float *T, int64_t ldt,
+                          float *X, int64_t ldx, void *stream);
+
 /* A = I (n x n, n <= 65535).  Replaces gen_identity_mat
  * (fp16_gpu_kernels.cpp:305-337). */
 int omx_identity_f32(int64_t n, float *A, int64_t lda, void *stream);

@@ -393,6 +393,43 @@ int omx_trsm_f32(int variant, int64_t m, int64_t n, const float *T, int64_t ldt,
     return OMX_ERR_ARG;
 }
 
+int omx_trsm_identity_f32(int variant, int64_t n, const float *T, int64_t ldt, float *X, int64_t ldx,
+                          void *stream) {
+    if (n <= 0 || n % kNB || n > 65535 || !T || !X || ldt < n || ldx < n) return OMX_ERR_ARG;
+    if (variant != OMX_TRSM_RUNN && variant != OMX_TRSM_RLNU) return OMX_ERR_ARG;
+    hipStream_t st = omx_stream(stream);
+    hipLaunchKernelGGL(k_identity, dim3((unsigned)omx_cdiv(n, 256), (unsigned)n), dim3(256), 0, st, n, X, ldx);
+    OMX_LAUNCH_CHECK();
+    if (variant == OMX_TRSM_RUNN) {
+        // block column j of I * U^-1 is zero below row j + 63
+        for (int64_t j = 0; j < n; j += kNB) {
+            const int64_t rows = j + kNB;
+            hipLaunchKernelGGL(k_trsm_rows_runn64, dim3((unsigned)omx_cdiv(rows, 64)), dim3(256), 0, st, rows,
+                               T + j + j * ldt, ldt, X + j * ldx, ldx);
+            OMX_LAUNCH_CHECK();
+            const int64_t rest = n - j - kNB;
+            if (rest > 0) {
+                int rc = sgemm_big(rows, rest, kNB, -1.0f, X + j * ldx, ldx, T + j + (j + kNB) * ldt, ldt, 1.0f,
+                                   X + (j + kNB) * ldx, ldx, st);
+                if (rc) return rc;
+            }
+        }
+        return OMX_OK;
+    }
+    // block column j of I * L^-1 is zero above row j
+    for (int64_t j = n - kNB; j >= 0; j -= kNB) {
+        const int64_t rows = n - j;
+        hipLaunchKernelGGL(k_trsm_rows_rlnu64, dim3((unsigned)omx_cdiv(rows, 256)), dim3(256), 0, st, rows,
+                           T + j + j * ldt, ldt, X + j + j * ldx, ldx);
+        OMX_LAUNCH_CHECK();
+        if (j > 0) {
+            int rc = sgemm_big(rows, j, kNB, -1.0f, X + j + j * ldx, ldx, T + j, ldt, 1.0f, X + j, ldx, st);
+            if (rc) return rc;
+        }
+    }
+    return OMX_OK;
+}
+
 int omx_identity_f32(int64_t n, float *A, int64_t lda, void *stream) {
     if (n <= 0 || !A || lda < n || n > 65535) return OMX_ERR_ARG;
     hipLaunchKernelGGL(k_identity, dim3((unsigned)omx_cdiv(n, 256), (unsigned)n), dim3(256), 0,

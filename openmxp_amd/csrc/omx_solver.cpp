@@ -457,14 +457,12 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
                 RC(omx_trsm_f32(OMX_TRSM_RUNN, rest, b, Akk, lda, Acol, lda, cs));
                 RC(omx_trsm_f32(OMX_TRSM_LLNU, b, rest, Akk, lda, Arow, lda, cs));
             } else {
-                RC(omx_identity_f32(b, Wup, b, cs));
-                RC(omx_trsm_f32(OMX_TRSM_RUNN, b, b, Akk, lda, Wup, b, cs));
+                RC(omx_trsm_identity_f32(OMX_TRSM_RUNN, b, Akk, lda, Wup, b, cs));
                 // Y(j, l) = W(l, j), upper part only for -alt 2
                 RC(omx_pack_f16(b, b, Wup, b, 1, s->alt == 2 ? OMX_MASK_K_LE_R : OMX_MASK_NONE, Wup_h, nullptr, cs));
                 RC(omx_pack_f16(rest, b, Acol, 1, lda, OMX_MASK_NONE, Tp, nullptr, cs));
                 RC(omx_gemm_f16_packed(rest, b, b, 1.0f, Tp, 0, Wup_h, 0, 0.0f, Acol, lda, cs));
-                RC(omx_identity_f32(b, Wlo, b, cs));
-                RC(omx_trsm_f32(OMX_TRSM_RLNU, b, b, Akk, lda, Wlo, b, cs));
+                RC(omx_trsm_identity_f32(OMX_TRSM_RLNU, b, Akk, lda, Wlo, b, cs));
                 // X(i, l) = W(i, l), strict lower + unit diagonal for -alt 2
                 RC(omx_pack_f16(b, b, Wlo, 1, b, s->alt == 2 ? OMX_MASK_R_GT_K_UNIT : OMX_MASK_NONE, Wlo_h, nullptr,
                                 cs));

@@ -114,3 +114,40 @@ def test_identity_and_copy():
     O = empty(40 * 50, torch.float32)
     assert L.omx_copy_block_f32(50, 40, ptr(Md), ldm, ptr(O), 50, None) == 0
     assert np.array_equal(colmajor_to_numpy(O, 50, 40), M)
+
+
+@pytest.mark.parametrize("variant,n", [(0, 256), (0, 640), (2, 256), (2, 640)])
+def test_trsm_identity_matches_identity_then_trsm(variant, n):
+    """The triangular inverse that solves only the rows that can be
+    non-zero equals the identity followed by the full TRSM, bit for bit
+    (numpy compares -0 and +0 as equal: the skipped rows hold the identity's
+    zeros where the full solve may have formed -0); the product with the
+    triangle is the identity to FP32 accuracy."""
+    torch = torch_cuda()
+    L = _L()
+    rng = np.random.default_rng(n + variant)
+    if variant == 0:
+        T = (np.triu(rng.uniform(-1, 1, (n, n))) + np.eye(n) * 8).astype(np.float32)
+    else:
+        T = (np.tril(rng.uniform(-1, 1, (n, n)), -1) / 8 + np.eye(n)).astype(np.float32)
+    Td, ldt = _fdev(T, n + 8)
+    Xa = empty(n * n, torch.float32)
+    Xb = empty(n * n, torch.float32)
+    assert L.omx_trsm_identity_f32(variant, n, ptr(Td), ldt, ptr(Xa), n, None) == 0
+    assert L.omx_identity_f32(n, ptr(Xb), n, None) == 0
+    assert L.omx_trsm_f32(variant, n, n, ptr(Td), ldt, ptr(Xb), n, None) == 0
+    a = colmajor_to_numpy(Xa, n, n, n)
+    assert np.array_equal(a, colmajor_to_numpy(Xb, n, n, n))
+    assert np.array_equal(a, np.triu(a) if variant == 0 else np.tril(a))
+    Tt = T.astype(np.float64) if variant == 0 else np.tril(T.astype(np.float64), -1) + np.eye(n)
+    assert np.max(np.abs(a.astype(np.float64) @ Tt - np.eye(n))) < 1e-4
+
+
+def test_trsm_identity_argument_errors():
+    torch = torch_cuda()
+    L = _L()
+    t = empty(128 * 128, torch.float32)
+    x = empty(128 * 128, torch.float32)
+    assert L.omx_trsm_identity_f32(1, 128, ptr(t), 128, ptr(x), 128, None) < 0   # LLNU not offered
+    assert L.omx_trsm_identity_f32(0, 100, ptr(t), 128, ptr(x), 128, None) < 0   # n % 64
+    assert L.omx_trsm_identity_f32(0, 128, ptr(t), 64, ptr(x), 128, None) < 0    # ldt < n
