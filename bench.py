@@ -57,6 +57,10 @@ def parse():
     ap.add_argument("--alt", type=int, default=1)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-learned", action="store_true", help="measure the plain method only")
+    ap.add_argument("--group", type=int, choices=(2, 4), default=4,
+                    help="panels per trailing update for the learned (FP8) solver")
+    ap.add_argument("--group-plain", type=int, choices=(2, 4), default=2,
+                    help="panels per trailing update for the plain FP16 solver")
     ap.add_argument("--fp8-all", action="store_true", help="also measure FP8 on every step")
     ap.add_argument("--cpu-n", type=int, default=10240, help="order of the CPU baseline sample")
     ap.add_argument("--cpu-b", type=int, default=256)
@@ -77,7 +81,7 @@ def log(msg):
 
 
 class GpuSolve:
-    def __init__(self, n, b, alt, fp8, policy=None):
+    def __init__(self, n, b, alt, fp8, policy=None, group=2):
         import numpy as np
         import torch
         from openmxp_amd import _lib
@@ -90,7 +94,8 @@ class GpuSolve:
         self.rhs = torch.empty(n, dtype=torch.float64, device="cuda")
         self.x = torch.empty(n, dtype=torch.float64, device="cuda")
         self.h = C.c_void_p()
-        rc = self.L.omx_solver_create(n, b, alt, 1 if fp8 else 0, C.byref(self.h))
+        flags = (_lib.SOLVER_FP8 if fp8 else 0) | (_lib.SOLVER_GROUP4 if group == 4 else 0)
+        rc = self.L.omx_solver_create(n, b, alt, flags, C.byref(self.h))
         if rc != 0:
             raise RuntimeError(f"omx_solver_create failed: {rc}")
         if policy is not None:
@@ -361,7 +366,7 @@ def main():
         del src, dst
         log("copied 1 GiB device to device (PMC calibration)")
     log(f"n={n} b={b} alt={args.alt}: generating and solving on cuda:0")
-    gs = GpuSolve(n, b, args.alt, fp8=False)
+    gs = GpuSolve(n, b, args.alt, fp8=False, group=args.group_plain)
     res16, fwd16 = measure(gs, args, "plain fp16 + IR")
     gs.close()
     del gs
@@ -369,7 +374,7 @@ def main():
     aL = a8 = None
     fwdL = fwd8 = None
     if policy is not None or args.fp8_all:
-        gs = GpuSolve(n, b, args.alt, fp8=True, policy=policy)
+        gs = GpuSolve(n, b, args.alt, fp8=True, policy=policy, group=args.group)
         if policy is not None:
             resL, fwdL = measure(gs, args, "learned + GMRES-IR", None, 1, True)
             aL = agg(resL, flops)

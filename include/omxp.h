@@ -412,6 +412,7 @@ typedef struct omx_solver omx_solver;
 
 #define OMX_SOLVER_FP8 1     /* flag of omx_solver_create: also allocate FP8 panels */
 #define OMX_SOLVER_SCALE16 2 /* flag: FP16 Schur panels with per-strip pow2 scales */
+#define OMX_SOLVER_GROUP4 4  /* flag: one trailing update per 4 panels (depth 4b) instead of per 2 */
 
 /* Create a solver for order n, block b (n % b == 0, b % 256 == 0), panel
  * method alt (0: FP32 TRSM, 1: FP16 GEMM with the TRSM-on-identity inverse,

@@ -87,6 +87,10 @@ class System(C.Structure):
                 ("rank", C.c_int64), ("sigma", vp), ("P", vp), ("Q", vp)]
 
 
+SOLVER_FP8 = 1      # omx_solver_create flags
+SOLVER_SCALE16 = 2
+SOLVER_GROUP4 = 4
+
 SYSTEM_LCG = 0
 SYSTEM_LOWRANK = 1
 

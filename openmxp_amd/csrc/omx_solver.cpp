@@ -8,21 +8,21 @@
 // and R_i.  Every block (I, J) with I, J > i receives -L_i[I] R_i[J]^T, in
 // the order i = 0, 1, 2, ...; the reference applies one panel per pass over
 // the trailing matrix.  Here the trailing matrix receives the panels in
-// pairs (2p, 2p+1): the FP32 trailing matrix is read and written once per
-// pair with one product of depth 2b instead of twice (its traffic, not the
-// MFMA work, bounds the update's epilogue).  The two panels of a pair share
-// one packed buffer (panel 2p in K columns [0, b), panel 2p+1 in [b, 2b)).
-// Pair p, with panels 2p and 2p+1 factored:
-//   bulk stream  A[2p+4:, 2p+4:] -= [L_2p L_2p+1] [R_2p R_2p+1]^T
-//   crit stream  block row/col 2p+2 -= pair p; factor panel 2p+2;
-//                block row/col 2p+3 -= pair p, then -= panel 2p+2;
-//                factor panel 2p+3
-// The crit stream waits for the bulk product of pair p-1 (it covered blocks
-// 2p+2 and 2p+3), the bulk product of pair p for the pair's two panels; so
-// the factorisation of the next pair's panels overlaps the trailing update
-// of the current pair.  A pair whose panels
-// were packed in different precisions (or in scaled FP16, whose scales are
-// per panel) is applied as two products of depth b.
+// groups of G (2, or 4 with OMX_SOLVER_GROUP4): the FP32 trailing matrix is
+// read and written once per group with one product of depth G b instead of
+// G times (its traffic, not the MFMA work, bounds the update's epilogue).
+// The panels of a group share one packed buffer (panel G g + j in K columns
+// [j b, (j+1) b)).  Group g, with panels G g .. G g + G - 1 factored:
+//   bulk stream  A[G(g+2):, G(g+2):] -= [L_Gg .. L_Gg+G-1] [R_Gg .. R_Gg+G-1]^T
+//   crit stream  for each block t of the next group, in order:
+//                block row/col t -= group g, then -= the next group's
+//                panels before t; factor panel t
+// The crit stream waits for the bulk product of group g-1 (it covered the
+// next group's blocks), the bulk product of group g for group g's panels; so
+// the factorisation of the next group's panels overlaps the trailing update
+// of the current group.  Runs of a group's panels packed in the same
+// precision are applied as one product; panels in scaled FP16 (per-panel
+// scales) one by one.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -61,11 +61,12 @@ struct omx_solver {
     int fp8 = 0;
     hipStream_t crit = nullptr, bulk = nullptr;
     std::vector<hipEvent_t> ev_panel, ev_bulk, ev_t0, ev_t1;
-    DevBuf Lp[2], Rp[2], Tp;            // FP16 pair buffers (K = 2b) and the -alt temporary
-    DevBuf L8[2], R8[2], Le[2], Re[2];  // FP8 pair buffers and their exponents
+    int64_t G = 2;                      // panels per trailing update (group)
+    DevBuf Lp[2], Rp[2], Tp;            // FP16 group buffers (K = G b) and the -alt temporary
+    DevBuf L8[2], R8[2], Le[2], Re[2];  // FP8 group buffers and their exponents
     DevBuf L8max, R8max;                // tile max-abs scratch
     int scale16 = 0;                    // FP16 Schur panels with per-strip scales
-    DevBuf Lse[4], Rse[4];              // their strip exponents, per (pair buffer, window)
+    DevBuf Lse[8], Rse[8];              // their strip exponents, per (group buffer, window)
     DevBuf Wup, Wlo, Wup_h, Wlo_h;      // -alt inverses (FP32, packed FP16)
     DevBuf info;
     // learned schedule: MLP parameters, per-step features, forward scratch
@@ -188,6 +189,7 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
     s->alt = alt;
     s->fp8 = (flags & OMX_SOLVER_FP8) ? 1 : 0;
     s->scale16 = (flags & OMX_SOLVER_SCALE16) ? 1 : 0;
+    s->G = (flags & OMX_SOLVER_GROUP4) ? 4 : 2;
     int rc = OMX_OK;
     auto fail = [&](int code) {
         omx_solver_destroy(s);
@@ -207,20 +209,21 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
         if (hipEventCreate(&s->ev_t1[k]) != hipSuccess) return fail(OMX_ERR_HIP);
     }
     const int64_t rows = (n > b) ? n - b : b;
-    // pair buffers: packed rows from the pair's first panel row, K = 2b
-    const size_t pb16 = omx_panel_bytes(rows, 2 * b, 0);
+    // group buffers: packed rows from the group's first panel row, K = G b
+    const int64_t kg = s->G * b;
+    const size_t pb16 = omx_panel_bytes(rows, kg, 0);
     for (int i = 0; i < 2; ++i) {
         if ((rc = dalloc(s->Lp[i], pb16)) || (rc = dalloc(s->Rp[i], pb16))) return fail(rc);
         if (s->fp8) {
-            const size_t pb8 = omx_panel_bytes(rows, 2 * b, 1);
-            const size_t ne = omx_panel_exps_count(rows, 2 * b);
+            const size_t pb8 = omx_panel_bytes(rows, kg, 1);
+            const size_t ne = omx_panel_exps_count(rows, kg);
             if ((rc = dalloc(s->L8[i], pb8)) || (rc = dalloc(s->R8[i], pb8)) ||
                 (rc = dalloc(s->Le[i], ne * 4)) || (rc = dalloc(s->Re[i], ne * 4)))
                 return fail(rc);
         }
     }
     if (s->scale16)
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 2 * s->G; ++i)
             if ((rc = dalloc(s->Lse[i], omx_strips(rows) * 4)) || (rc = dalloc(s->Rse[i], omx_strips(rows) * 4)))
                 return fail(rc);
     if (s->fp8 || s->scale16) {
@@ -237,7 +240,7 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
     s->last_sched.assign(nb, 0);
     s->prec.assign(nb, 0);
     s->last_feat.assign((size_t)nb * OMX_MLP_NFEATURES, 0.0f);
-    if ((rc = dalloc(s->Tp, pb16))) return fail(rc);
+    if ((rc = dalloc(s->Tp, omx_panel_bytes(rows, b, 0)))) return fail(rc);
     if ((rc = dalloc(s->Wup, (size_t)b * b * 4)) || (rc = dalloc(s->Wlo, (size_t)b * b * 4)) ||
         (rc = dalloc(s->Wup_h, omx_panel_bytes(b, b, 0))) || (rc = dalloc(s->Wlo_h, omx_panel_bytes(b, b, 0))))
         return fail(rc);
@@ -262,9 +265,12 @@ int omx_solver_destroy(omx_solver *s) {
                       &s->R8[1], &s->Le[0], &s->Le[1], &s->Re[0], &s->Re[1], &s->L8max, &s->R8max,
                       &s->Wup, &s->Wlo, &s->Wup_h, &s->Wlo_h, &s->info, &s->r,
                       &s->w, &s->gemv_ws, &s->trsv_ws, &s->nrm, &s->V, &s->hv, &s->dots,
-                      &s->pol, &s->feat, &s->mlp_h, &s->zdev, &s->Lse[0], &s->Lse[1], &s->Lse[2],
-                      &s->Lse[3], &s->Rse[0], &s->Rse[1], &s->Rse[2], &s->Rse[3]};
+                      &s->pol, &s->feat, &s->mlp_h, &s->zdev};
     for (DevBuf *d : bufs) dfree(*d);
+    for (int i = 0; i < 8; ++i) {
+        dfree(s->Lse[i]);
+        dfree(s->Rse[i]);
+    }
     if (s->ev_dec) (void)hipEventDestroy(s->ev_dec);
     if (s->zhost) (void)hipHostFree(s->zhost);
     if (s->crit) (void)hipStreamDestroy(s->crit);
@@ -273,20 +279,20 @@ int omx_solver_destroy(omx_solver *s) {
     return OMX_OK;
 }
 
-// Packed coordinates of panel i: pair buffer (i/2) % 2, K window (i % 2) b,
-// packed row 0 = matrix row (2 (i/2) + 1) b, the first row of the pair's
-// first column panel.
-static inline int pbuf(int64_t i) { return (int)((i >> 1) & 1); }
-static inline int64_t porigin(int64_t i, int64_t b) { return ((i & ~(int64_t)1) + 1) * b; }
-static inline int64_t pwin(int64_t i, int64_t b) { return (i & 1) * b; }
+// Packed coordinates of panel i in groups of G: group buffer (i/G) % 2,
+// K window (i % G) b, packed row 0 = matrix row (G (i/G) + 1) b, the first
+// row of the group's first column panel.
+static inline int pbuf(int64_t i, int64_t G) { return (int)((i / G) & 1); }
+static inline int64_t porigin(int64_t i, int64_t b, int64_t G) { return ((i / G) * G + 1) * b; }
+static inline int64_t pwin(int64_t i, int64_t b, int64_t G) { return (i % G) * b; }
 
-// A(r0:r0+m, c0:c0+nn) -= L_i R_i^T over K window [k0, k0 + k) of panel i's
-// pair buffer (k = 2b, k0 = 0: both panels of the pair).
+// A(r0:r0+m, c0:c0+nn) -= L R^T over K window [k0, k0 + k) of panel i's
+// group buffer (the panels whose windows it spans, from panel i on).
 static int schur_window(omx_solver *s, int64_t i, int64_t k, int64_t k0, int64_t r0, int64_t c0, int64_t m,
                         int64_t nn, float *A, int64_t lda, hipStream_t st) {
-    const int64_t b = s->b, ldk = 2 * b;
-    const int buf = pbuf(i);
-    const int64_t xr0 = r0 - porigin(i, b), yr0 = c0 - porigin(i, b);
+    const int64_t b = s->b, G = s->G, ldk = G * b;
+    const int buf = pbuf(i, G);
+    const int64_t xr0 = r0 - porigin(i, b, G), yr0 = c0 - porigin(i, b, G);
     float *C = A + r0 + c0 * lda;
     if (s->prec[i])
         return omx_gemm_f8_packed_window(m, nn, k, -1.0f, static_cast<uint8_t *>(s->L8[buf].p),
@@ -295,23 +301,28 @@ static int schur_window(omx_solver *s, int64_t i, int64_t k, int64_t k0, int64_t
                                          yr0, ldk, k0, 1.0f, C, lda, st);
     const int32_t *xe = nullptr, *ye = nullptr;
     if (s->scale16) {
-        xe = static_cast<int32_t *>(s->Lse[2 * buf + (i & 1)].p);
-        ye = static_cast<int32_t *>(s->Rse[2 * buf + (i & 1)].p);
+        xe = static_cast<int32_t *>(s->Lse[G * buf + i % G].p);
+        ye = static_cast<int32_t *>(s->Rse[G * buf + i % G].p);
     }
     return omx_gemm_f16_packed_window(m, nn, k, -1.0f, static_cast<uint16_t *>(s->Lp[buf].p), xe, xr0, ldk, k0,
                                       static_cast<uint16_t *>(s->Rp[buf].p), ye, yr0, ldk, k0, 1.0f, C, lda, st);
 }
 
-// A(r0.., c0..) (m x nn) -= the panels i0 .. i1 (i1 = i0 or i0 + 1), in
-// order; one product of depth 2b when they are a pair packed alike.
+// A(r0.., c0..) (m x nn) -= the panels i0 .. i1 (all in one group), in
+// order; each run of consecutive panels packed alike is one product over
+// its K windows (scaled FP16 panels, whose scales are per panel, one by one).
 static int apply_panels(omx_solver *s, int64_t i0, int64_t i1, int64_t r0, int64_t c0, int64_t m, int64_t nn,
                         float *A, int64_t lda, hipStream_t st) {
     if (m <= 0 || nn <= 0) return OMX_OK;
-    const int64_t b = s->b;
-    const bool fuse = i1 == i0 + 1 && (i0 & 1) == 0 && (s->prec[i0] > 0) == (s->prec[i1] > 0) &&
-                      !(s->scale16 && s->prec[i0] == 0);
-    if (fuse) return schur_window(s, i0, 2 * b, 0, r0, c0, m, nn, A, lda, st);
-    for (int64_t i = i0; i <= i1; ++i) RC(schur_window(s, i, b, pwin(i, b), r0, c0, m, nn, A, lda, st));
+    const int64_t b = s->b, G = s->G;
+    for (int64_t i = i0; i <= i1;) {
+        int64_t j = i;
+        const bool f8 = s->prec[i] > 0;
+        if (f8 || !s->scale16)
+            while (j + 1 <= i1 && (j + 1) / G == i / G && (s->prec[j + 1] > 0) == f8) ++j;
+        RC(schur_window(s, i, (j - i + 1) * b, pwin(i, b, G), r0, c0, m, nn, A, lda, st));
+        i = j + 1;
+    }
     return OMX_OK;
 }
 
@@ -324,14 +335,14 @@ static int update_block(omx_solver *s, int64_t t, int64_t i0, int64_t i1, float 
     return OMX_OK;
 }
 
-// Pack panel i (its finished FP32 column / row panels) into its pair buffer
-// window, in the precision s->prec[i].
+// Pack panel i (its finished FP32 column / row panels) into its group
+// buffer window, in the precision s->prec[i].
 static int pack_panel(omx_solver *s, int64_t i, int have_max, int64_t rest, const float *Acol, const float *Arow,
                       int64_t lda, hipStream_t st) {
-    const int64_t b = s->b, ldk = 2 * b;
-    const int buf = pbuf(i);
-    const int64_t rstrip = (pwin(i, b) + 0) / OMX_STRIP;  // packed rows start at (i % 2) b
-    const int64_t k0 = pwin(i, b);
+    const int64_t b = s->b, G = s->G, ldk = G * b;
+    const int buf = pbuf(i, G);
+    const int64_t rstrip = pwin(i, b, G) / OMX_STRIP;  // packed rows start at (i % G) b
+    const int64_t k0 = pwin(i, b, G);
     if (s->prec[i]) {
         const int h = s->prec[i] - 1;
         const size_t sb8 = (size_t)(ldk / OMX_KSTAGE8) * 32768, ne = (size_t)(ldk / OMX_SCALE_TILE);
@@ -350,8 +361,8 @@ static int pack_panel(omx_solver *s, int64_t i, int have_max, int64_t rest, cons
     uint16_t *Rp = static_cast<uint16_t *>(s->Rp[buf].p) + rstrip * sb16;
     int32_t *xe = nullptr, *ye = nullptr;
     if (s->scale16) {
-        xe = static_cast<int32_t *>(s->Lse[2 * buf + (i & 1)].p) + rstrip;
-        ye = static_cast<int32_t *>(s->Rse[2 * buf + (i & 1)].p) + rstrip;
+        xe = static_cast<int32_t *>(s->Lse[G * buf + i % G].p) + rstrip;
+        ye = static_cast<int32_t *>(s->Rse[G * buf + i % G].p) + rstrip;
     }
     RC(omx_pack_f16_window(rest, b, Acol, 1, lda, have_max, Lp, xe, static_cast<float *>(s->L8max.p), ldk, k0, st));
     RC(omx_pack_f16_window(rest, b, Arow, lda, 1, have_max, Rp, ye, static_cast<float *>(s->R8max.p), ldk, k0, st));
@@ -485,40 +496,38 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
         return OMX_OK;
     };
 
-    // panels 0 and 1
-    RC(factor(0));
-    if (nb > 1) {
-        RC(update_block(s, 1, 0, 0, A, lda, cs));
-        RC(factor(1));
+    // group 0's panels
+    const int64_t G = s->G;
+    for (int64_t t = 0; t < G && t < nb; ++t) {
+        if (t > 0) RC(update_block(s, t, 0, t - 1, A, lda, cs));
+        RC(factor(t));
     }
     OMX_HIP_CHECK(hipEventRecord(s->ev_panel[0], cs));
-    for (int64_t p = 0; 2 * p + 2 < nb; ++p) {
-        const int64_t i0 = 2 * p, i1 = 2 * p + 1;
-        // ---- bulk: the pair's trailing update beyond blocks 2p+2, 2p+3 ----
-        const int64_t f = 2 * p + 4;
+    for (int64_t g = 0; G * (g + 1) < nb; ++g) {
+        const int64_t i0 = G * g, i1 = std::min(G * g + G - 1, nb - 1);
+        // ---- bulk: the group's trailing update beyond the next group ----
+        const int64_t f = G * (g + 2);
         if (f < nb) {
             const int64_t m = n - f * b;
-            OMX_HIP_CHECK(hipStreamWaitEvent(bs, s->ev_panel[p], 0));
+            OMX_HIP_CHECK(hipStreamWaitEvent(bs, s->ev_panel[g], 0));
             OMX_HIP_CHECK(hipEventRecord(s->ev_t0[s->bulk_count], bs));
             RC(apply_panels(s, i0, i1, f * b, f * b, m, m, A, lda, bs));
             OMX_HIP_CHECK(hipEventRecord(s->ev_t1[s->bulk_count], bs));
-            s->bulk_flops += 4.0 * (double)m * m * b;
+            s->bulk_flops += 2.0 * (double)m * m * (double)((i1 - i0 + 1) * b);
             ++s->bulk_count;
         }
-        OMX_HIP_CHECK(hipEventRecord(s->ev_bulk[p], bs));
-        // ---- crit: the next pair's panels ----
-        // blocks 2p+2 and 2p+3 hold the previous pairs' updates once the
-        // previous bulk product is done
-        if (p >= 1) OMX_HIP_CHECK(hipStreamWaitEvent(cs, s->ev_bulk[p - 1], 0));
-        const int64_t t0b = 2 * p + 2, t1b = 2 * p + 3;
-        RC(update_block(s, t0b, i0, i1, A, lda, cs));
-        RC(factor(t0b));
-        if (t1b < nb) {
-            RC(update_block(s, t1b, i0, i1, A, lda, cs));
-            RC(update_block(s, t1b, t0b, t0b, A, lda, cs));
-            RC(factor(t1b));
+        OMX_HIP_CHECK(hipEventRecord(s->ev_bulk[g], bs));
+        // ---- crit: the next group's panels ----
+        // its blocks hold the earlier groups' updates once the previous bulk
+        // product is done
+        if (g >= 1) OMX_HIP_CHECK(hipStreamWaitEvent(cs, s->ev_bulk[g - 1], 0));
+        const int64_t h0 = G * (g + 1), h1 = std::min(h0 + G - 1, nb - 1);
+        for (int64_t t = h0; t <= h1; ++t) {
+            RC(update_block(s, t, i0, i1, A, lda, cs));
+            if (t > h0) RC(update_block(s, t, h0, t - 1, A, lda, cs));
+            RC(factor(t));
         }
-        OMX_HIP_CHECK(hipEventRecord(s->ev_panel[p + 1], cs));
+        OMX_HIP_CHECK(hipEventRecord(s->ev_panel[g + 1], cs));
     }
     OMX_HIP_CHECK(hipStreamSynchronize(cs));
     OMX_HIP_CHECK(hipStreamSynchronize(bs));

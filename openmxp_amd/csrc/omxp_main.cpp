@@ -28,6 +28,7 @@ struct Opts {
     long n = 61440, b = 2560, p = 1, q = 1;
     int solv = 1, alt = 1, log = 1, progress = 10, init = 1, skip = 0;
     int ir = 0, maxit = 50, restart = 50, device = 0;
+    int group = 0;  // panels per trailing update: 0 = 4 for fp8/learned, 2 for fp16
     double inner_tol = 1e-10;
     unsigned long long seed = 42;
     std::string prec = "fp16";
@@ -38,7 +39,7 @@ void usage() {
     std::printf(
         "usage: omxp N B [P Q] [-solv 0|1] [-alt 0|1|2] [-log 0|1] [-progress k] [-init 1]\n"
         "            [-prec fp16|fp8|learned] [-policy file] [-ir classic|gmres] [-seed s] [-maxit k]\n"
-        "            [-restart m] [-device d]\n");
+        "            [-restart m] [-device d] [-group 2|4]\n");
 }
 
 bool parse(int argc, char **argv, Opts &o) {
@@ -72,6 +73,7 @@ bool parse(int argc, char **argv, Opts &o) {
         else if (!std::strcmp(a, "-maxit")) o.maxit = std::atoi(next(a));
         else if (!std::strcmp(a, "-restart")) o.restart = std::atoi(next(a));
         else if (!std::strcmp(a, "-device")) o.device = std::atoi(next(a));
+        else if (!std::strcmp(a, "-group")) o.group = std::atoi(next(a));
         else if (!std::strcmp(a, "-h") || !std::strcmp(a, "--help")) {
             usage();
             std::exit(0);
@@ -123,6 +125,10 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "-prec must be fp16, fp8 or learned\n");
         return 2;
     }
+    if (o.group != 0 && o.group != 2 && o.group != 4) {
+        std::fprintf(stderr, "-group must be 2 or 4\n");
+        return 2;
+    }
     const bool learned = (o.prec == "learned");
     const bool fp8 = (o.prec == "fp8") || learned;
     std::vector<float> params;
@@ -158,7 +164,8 @@ int main(int argc, char **argv) {
     CK(hipMalloc(&rhs, sizeof(double) * n));
     CK(hipMalloc(&x, sizeof(double) * n));
     omx_solver *s = nullptr;
-    if (omx_solver_create(n, b, o.alt, fp8 ? OMX_SOLVER_FP8 : 0, &s) != 0) {
+    const int group = o.group ? o.group : (fp8 ? 4 : 2);
+    if (omx_solver_create(n, b, o.alt, (fp8 ? OMX_SOLVER_FP8 : 0) | (group == 4 ? OMX_SOLVER_GROUP4 : 0), &s) != 0) {
         std::fprintf(stderr, "solver creation failed\n");
         return 1;
     }

@@ -23,7 +23,7 @@ def test_help():
 
 
 @pytest.mark.parametrize("args", [("1024", "256", "2", "2"), ("1024", "256", "-skip", "1"),
-                                  ("1024", "256", "-prec", "bf16"), ("1024", "100"),
+                                  ("1024", "256", "-prec", "bf16"), ("1024", "100"), ("1024", "256", "-group", "3"),
                                   ("1024", "256", "-alt", "3"),
                                   ("1024", "256", "-prec", "learned", "-policy", "/nonexistent/p.f32")])
 def test_argument_errors_exit_2(args):
@@ -39,7 +39,8 @@ def _harness(out):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("extra", [(), ("-alt", "0"), ("-alt", "2", "-ir", "gmres"), ("-prec", "fp8", "-ir", "gmres")])
+@pytest.mark.parametrize("extra", [(), ("-alt", "0"), ("-alt", "2", "-ir", "gmres"), ("-prec", "fp8", "-ir", "gmres"),
+                                   ("-group", "4"), ("-prec", "fp8", "-group", "2", "-ir", "gmres")])
 def test_solve_small(extra):
     from tests.gpu_util import torch_cuda
     torch_cuda()

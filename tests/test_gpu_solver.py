@@ -18,10 +18,11 @@ def _L():
 
 
 class Solver:
-    def __init__(self, n, b, alt=1, fp8=False):
+    def __init__(self, n, b, alt=1, fp8=False, group4=False):
         self.L = _L()
         self.h = C.c_void_p()
-        rc = self.L.omx_solver_create(n, b, alt, 1 if fp8 else 0, C.byref(self.h))
+        flags = (1 if fp8 else 0) | (4 if group4 else 0)
+        rc = self.L.omx_solver_create(n, b, alt, flags, C.byref(self.h))
         assert rc == 0, rc
         self.n, self.b = n, b
 
@@ -211,3 +212,58 @@ def test_scaled_fp16_lu_matches_plain():
     assert np.abs(outs[0] - outs[1])[low].max() <= 2.0**-8 * np.abs(outs[0][low]).max()
     print(f"  backward error plain {berr[0]:.3e} scaled {berr[1]:.3e}", flush=True)
     assert berr[1] <= 1.05 * berr[0]
+
+
+@pytest.mark.parametrize("n,alt", [(2560, 1), (3584, 0), (3584, 1)])
+def test_lu_group4_matches_oracle(orc, n, alt):
+    """OMX_SOLVER_GROUP4: the trailing matrix takes one product per four
+    panels (depth 4b); the factors agree with the oracle's reference-order
+    LU to the FP16-update tolerance, also with a last group of two panels
+    (n / b = 10, 14)."""
+    torch = torch_cuda()
+    b, seed = 256, 42
+    print(f"lu group4 n={n} b={b} alt={alt}", flush=True)
+    lda = n + 64
+    A, d, rhs = _generate(n, seed, lda)
+    S = Solver(n, b, alt, group4=True)
+    info, tim = S.lu(A, lda)
+    S.close()
+    assert info == 0
+    got = colmajor_to_numpy(A, n, n, lda).astype(np.float64)
+    Ao, _ = orc.matgen_full(n, seed)
+    want, _ = orc.lu_factor(Ao, b, alt=alt)
+    want = want.astype(np.float64)
+    low = np.tril(np.ones((n, n)), -1).astype(bool)
+    dl = np.abs(got - want)[low]
+    du = np.abs(got - want)[~low]
+    assert dl.max() <= 2.0**-8 * np.abs(want[low]).max()
+    assert np.max(du / (np.abs(want[~low]) + 0.05)) <= 2.0**-8
+    # one bulk product per group g while blocks 4 (g + 2) .. exist
+    nb = n // b
+    assert tim[2] == sum(1 for g in range(nb) if 4 * (g + 1) < nb and 4 * (g + 2) < nb)
+
+
+def test_lu_group4_mixed_precision_schedule():
+    """Groups whose panels mix FP8 and FP16 are applied as one product per
+    run of equal precision: the factors match the pairwise solver's under
+    the same schedule to FP8 accuracy, and the GMRES refinement converges."""
+    torch = torch_cuda()
+    n, b, seed = 3584, 256, 5
+    sched = np.array([1, 0, 1, 1, 0, 0, 1, 0, 1, 1, 1, 0, 0, 1], dtype=np.int32)
+    facs = []
+    for g4 in (False, True):
+        A, d, rhs = _generate(n, seed, n)
+        S = Solver(n, b, 1, fp8=True, group4=g4)
+        info, _ = S.lu(A, n, sched=sched)
+        assert info == 0
+        x = empty(n, torch.float64)
+        rc, out, _ = S.ir(A, n, seed, d, rhs, x, method=1)
+        S.close()
+        print(f"  group4={g4}: rc {rc} corrections {int(out[2])} inner {int(out[3])} harness {out[1]:.4f}",
+              flush=True)
+        assert rc == 0 and out[1] < 16.0
+        facs.append(colmajor_to_numpy(A, n, n).astype(np.float64))
+    low = np.tril(np.ones((n, n), dtype=bool), -1)
+    d2 = np.abs(facs[0] - facs[1])
+    assert d2[low].max() <= 2.0**-4 * np.abs(facs[0][low]).max()
+    assert np.max(d2[~low] / (np.abs(facs[0][~low]) + 0.05)) <= 2.0**-4
