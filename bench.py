@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (the reference kernel for PMC byte counters)")
     ap.add_argument("--gemm-sweep", action="store_true",
-                    help="run only the FP16 paired trailing-update GEMMs of one factorisation (m x m x 2b, m = n-(2p+4)b) "
+                    help="run only the trailing-update GEMMs of one factorisation, FP16 in pairs then FP8 in groups of four "
                          "on the default stream, for counter collection")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the Schur GEMM traffic from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
@@ -248,9 +248,8 @@ def summarize_pmc(fetch_csv, write_csv, n, b):
         # the copy reads and writes 1 GiB each
         cal[counter] = copy_bytes / c[0]
         data[counter] = rs
-    ms = bulk_shapes(n, b)
-    print(f"workload: the paired trailing updates of one factorisation, n={n} b={b} (m x m x 2b, "
-          f"m = n - (2p+4) b, {len(ms)} launches per precision)")
+    print(f"workload: the trailing updates of one factorisation, n={n} b={b}: FP16 in pairs (m x m x 2b, "
+          f"m = n - (2p+4) b), FP8 in groups of four (m x m x 4b, m = n - (4g+8) b)")
     print(f"fetch_calibration_factor: {cal['FETCH_SIZE']:.4f}  (bytes per counted byte, 1 GiB copy)")
     print(f"write_calibration_factor: {cal['WRITE_SIZE']:.4f}")
     for tag, kname, eb in (("f16", "k_gemm_f16<false", 2.0), ("f8", "k_gemm_f8<false", 1.0)):
@@ -259,51 +258,61 @@ def summarize_pmc(fetch_csv, write_csv, n, b):
         if not fetch or not write:
             continue
         nl = len(fetch)
-        # algorithmic bytes: FP32 C read + write, the two packed panels of K = 2b
-        alg = sum(8.0 * m * m + 2 * eb * m * 2 * b for m in ms)
+        G = SWEEP_GROUPS["fp16" if tag == "f16" else "fp8"]
+        ms = bulk_shapes(n, b, G)
+        # algorithmic bytes: FP32 C read + write, the two packed panels of K = G b
+        alg = sum(8.0 * m * m + 2 * eb * m * G * b for m in ms)
         print(f"{tag}.kernel: {kname},4> (trailing update)")
         print(f"{tag}.launches: {nl} (algorithmic model: {len(ms)})")
         print(f"{tag}.fetch_bytes_total: {sum(fetch):.4e}")
         print(f"{tag}.write_bytes_total: {sum(write):.4e}")
-        print(f"{tag}.algorithmic_bytes_total: {alg:.4e}  (8 m^2 for C + {2 * eb:.0f} m 2b for the two panels)")
+        print(f"{tag}.algorithmic_bytes_total: {alg:.4e}  (8 m^2 for C + {2 * eb:.0f} m {G}b for the two panels)")
         print(f"{tag}.traffic_over_algorithmic: {(sum(fetch) + sum(write)) / alg:.3f}")
         print(f"{tag}.traffic_bytes_per_launch: {(sum(fetch) + sum(write)) / max(nl, 1):.6e}")
 
 
-def bulk_shapes(n, b):
-    """Row counts m of the paired trailing updates of one factorisation:
-    panels 2p and 2p+1 update A[f:, f:] with f = (2p + 4) b, K = 2 b."""
+def bulk_shapes(n, b, G=2):
+    """Row counts m of the trailing updates of one factorisation with panels
+    in groups of G: group g updates A[f:, f:] with f = G (g + 2) b, depth
+    K = G b."""
     nb = n // b
     out = []
-    p = 0
-    while 2 * p + 2 < nb:
-        m = n - (2 * p + 4) * b
-        if m > 0:
-            out.append(m)
-        p += 1
+    g = 0
+    while G * (g + 1) < nb:
+        if G * (g + 2) < nb:
+            out.append(n - G * (g + 2) * b)
+        g += 1
     return out
 
 
+# the solvers the bench measures: plain FP16 in pairs, the learned FP8
+# schedule in groups of four
+SWEEP_GROUPS = {"fp16": 2, "fp8": 4}
+
+
 def gemm_sweep(n, b, seed, precs=("fp16", "fp8")):
-    """The paired trailing updates of one factorisation (m x m x 2b,
-    m = n - (2p+4) b), one launch each, on the null stream, first with FP16
-    then with FP8 packed panels of the right size (random contents) and the
-    generated matrix as C; prints the event-timed rates."""
+    """The trailing updates of one factorisation as the bench's solvers make
+    them (FP16: pairs, m x m x 2b; FP8: groups of four, m x m x 4b), one
+    launch each, on the null stream, with packed panels of the right size
+    (random contents) and the generated matrix as C; prints the event-timed
+    rates."""
     import torch
     from openmxp_amd import _lib
     L = _lib.lib()
     A = torch.empty(n * n, dtype=torch.float32, device="cuda")
     d = torch.empty(n, dtype=torch.float64, device="cuda")
     assert L.omx_matgen_f32(n, seed, A.data_ptr(), n, d.data_ptr(), None) == 0
-    ms = bulk_shapes(n, b)
     for prec in precs:
         fp8 = prec == "fp8"
-        pb = L.omx_panel_bytes(n - 2 * b, 2 * b, 1 if fp8 else 0)
+        G = SWEEP_GROUPS[prec]
+        ms = bulk_shapes(n, b, G)
+        k = G * b
+        pb = L.omx_panel_bytes(n - b, k, 1 if fp8 else 0)
         if fp8:
             # e4m3 codes below 0x78 (no NaN), unit scales
             X = torch.randint(0, 0x78, (pb,), dtype=torch.uint8, device="cuda")
             Y = torch.randint(0, 0x78, (pb,), dtype=torch.uint8, device="cuda")
-            ex = torch.zeros(L.omx_panel_exps_count(n - 2 * b, 2 * b), dtype=torch.int32, device="cuda")
+            ex = torch.zeros(L.omx_panel_exps_count(n - b, k), dtype=torch.int32, device="cuda")
         else:
             X = (torch.rand(pb // 2, device="cuda") - 0.5).half()
             Y = (torch.rand(pb // 2, device="cuda") - 0.5).half()
@@ -315,16 +324,17 @@ def gemm_sweep(n, b, seed, precs=("fp16", "fp8")):
             f = n - m
             Cp = A.data_ptr() + 4 * f * (n + 1)
             if fp8:
-                rc = L.omx_gemm_f8_packed(m, m, 2 * b, -1.0, X.data_ptr(), ex.data_ptr(), 0, Y.data_ptr(),
+                rc = L.omx_gemm_f8_packed(m, m, k, -1.0, X.data_ptr(), ex.data_ptr(), 0, Y.data_ptr(),
                                           ex.data_ptr(), 0, 1.0, Cp, n, None)
             else:
-                rc = L.omx_gemm_f16_packed(m, m, 2 * b, -1.0, X.data_ptr(), 0, Y.data_ptr(), 0, 1.0, Cp, n, None)
+                rc = L.omx_gemm_f16_packed(m, m, k, -1.0, X.data_ptr(), 0, Y.data_ptr(), 0, 1.0, Cp, n, None)
             assert rc == 0
-            flops += 2.0 * m * m * 2 * b
+            flops += 2.0 * m * m * k
         e1.record()
         torch.cuda.synchronize()
         t = e0.elapsed_time(e1) * 1e-3
-        log(f"gemm sweep {prec} n={n} b={b}: {len(ms)} launches, {t * 1e3:.1f} ms, {flops / t * 1e-12:.1f} TFLOP/s")
+        log(f"gemm sweep {prec} n={n} b={b} depth {k}: {len(ms)} launches, {t * 1e3:.1f} ms, "
+            f"{flops / t * 1e-12:.1f} TFLOP/s")
         del X, Y
 
 
@@ -391,7 +401,8 @@ def main():
     head = aL if aL else a16
     roof16 = {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
               "frac": round(achieved / PEAK_FP16_TFLOPS, 4), "traffic": pmc_traffic("f16"),
-              "kernel": "k_gemm_f16 paired trailing update (alpha=-1, beta=1), plain run",
+              "kernel": f"k_gemm_f16 trailing update per {args.group_plain} panels (depth {args.group_plain}b, "
+                        "alpha=-1, beta=1), plain run",
               "launches": a16["gemm_launches"]}
     line = {
         "metric": "LU+IR TFLOP/s @N=204800 1 GPU, learned FP8/FP16 vs plain FP16; resid<16",
@@ -430,7 +441,8 @@ def main():
         # the schedule chose FP8; all of them when fp8_steps == steps)
         line["roofline"] = {"bound": "mfma", "achieved": round(achL, 1), "peak": PEAK_FP8_TFLOPS, "unit": "TFLOP/s",
                             "frac": round(achL / PEAK_FP8_TFLOPS, 4), "traffic": pmc_traffic("f8"),
-                            "kernel": "k_gemm_f8 paired trailing update (alpha=-1, beta=1), learned run",
+                            "kernel": f"k_gemm_f8 trailing update per {args.group} panels (depth {args.group}b, "
+                                      "alpha=-1, beta=1), learned run",
                             "launches": aL["gemm_launches"],
                             "fp8_steps": aL["fp8_steps"], "steps": aL["steps_total"]}
         line["learned"] = {"speedup_vs_plain_fp16": round(aL["tflops"] / a16["tflops"], 3),
