@@ -347,6 +347,20 @@ int omx_trsm_f32(int variant, int64_t m, int64_t n, const float *T,
 int omx_trsm_identity_f32(int variant, int64_t n, const float *T, int64_t ldt,
                           float *X, int64_t ldx, void *stream);
 
+/* Both inverses of a factored no-pivot LU block D = L U (n x n, n a
+ * multiple of 64, at most 65535; L unit lower in D's strict lower part, U
+ * upper): Xu = U^-1 and Xl = L^-1 (ldx >= n; the other triangle of each is
+ * zero, Xl's diagonal 1).  The 64 x 64 diagonal blocks are inverted by
+ * substitution, then adjacent ranges are merged level by level with two
+ * batched FP32 products each (X12 = -X1 U12 X2, X21 = -X2 L21 X1): about
+ * 2 log2(n/64) launches instead of the 2 n/64 dependent steps of a solve
+ * against the identity.  work: n x n floats.  Replaces the two GPUBLAS_STRSM
+ * solves of the identity in the -alt 1/2 panel updates
+ * (panel_trf_gdirect.hpp:87-92, :120-125); the result differs from those
+ * solves by FP32 rounding only. */
+int omx_trtri_lu_f32(int64_t n, const float *D, int64_t ldd, float *Xu, float *Xl,
+                     int64_t ldx, float *work, void *stream);
+
 /* A = I (n x n, n <= 65535).  Replaces gen_identity_mat
  * (fp16_gpu_kernels.cpp:305-337). */
 int omx_identity_f32(int64_t n, float *A, int64_t lda, void *stream);

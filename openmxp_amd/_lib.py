@@ -62,6 +62,7 @@ SIGNATURES = {
     "omx_getrf_nopiv_f32": (i32, [i64, vp, i64, vp, vp]),
     "omx_trsm_f32": (i32, [i32, i64, i64, vp, i64, vp, i64, vp]),
     "omx_trsm_identity_f32": (i32, [i32, i64, vp, i64, vp, i64, vp]),
+    "omx_trtri_lu_f32": (i32, [i64, vp, i64, vp, vp, i64, vp, vp]),
     "omx_identity_f32": (i32, [i64, vp, i64, vp]),
     "omx_copy_block_f32": (i32, [i64, i64, vp, i64, vp, i64, vp]),
     "omx_mlp_forward_f32": (i32, [i64, vp, vp, vp, vp, vp, vp]),

@@ -10,6 +10,7 @@
 // (sgetrf_nopiv.cpp:27-61) and as reference BLAS strsm does for a non-unit
 // diagonal.
 #include <utility>
+#include <vector>
 
 #include "omx_common.hpp"
 
@@ -24,9 +25,14 @@ constexpr int kNB = 64;
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_sgemm_nn(int64_t M, int64_t N, int64_t K, float alpha, const float *__restrict__ A, int64_t lda,
-           const float *__restrict__ B, int64_t ldb, float beta, float *__restrict__ C, int64_t ldc) {
+           const float *__restrict__ B, int64_t ldb, float beta, float *__restrict__ C, int64_t ldc,
+           int64_t sA, int64_t sB, int64_t sC) {
     __shared__ float As[16][64 + 4];
     __shared__ float Bs[16][64 + 4];
+    // batch blockIdx.z: operands sA / sB / sC floats further on
+    A += (int64_t)blockIdx.z * sA;
+    B += (int64_t)blockIdx.z * sB;
+    C += (int64_t)blockIdx.z * sC;
     const int tid = threadIdx.x;
     const int tx = tid & 15, ty = tid >> 4;
     const int64_t i0 = (int64_t)blockIdx.x * 64, j0 = (int64_t)blockIdx.y * 64;
@@ -264,6 +270,44 @@ k_trsm_rows_rlnu64(int64_t m, const float *__restrict__ L, int64_t ldl, float *_
     for (int j = 0; j < kNB; ++j) X[i + j * ldx] = x[j];
 }
 
+// Inverses of the 64 x 64 diagonal blocks of the factored block D (L unit
+// lower, U upper in one array): workgroup w < nblk writes U_ww^-1 into Xu
+// (row r of it = e_r U_ww^-1, a RUNN row solve of the identity row by a lane
+// quad), workgroup nblk + w writes L_ww^-1 into Xl (column c = L_ww^-1 e_c,
+// an LLNU column solve by a lane quad).
+__global__ void __launch_bounds__(256)
+k_tri_inv64(int nblk, const float *__restrict__ D, int64_t ldd, float *__restrict__ Xu, float *__restrict__ Xl,
+            int64_t ldx) {
+    __shared__ float t[kNB][kNB + 1];
+    __shared__ float rd[kNB];
+    const bool up = (int)blockIdx.x < nblk;
+    const int64_t o = (int64_t)(up ? blockIdx.x : blockIdx.x - nblk) * kNB;
+    load_tri64(D + o + o * ldd, ldd, t);
+    const int q = threadIdx.x & 3, r = threadIdx.x >> 2;
+    float v[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) v[s] = (q + 4 * s == r) ? 1.0f : 0.0f;
+    if (up) {
+        diag_recip64(t, rd);
+        runn_all(v, q, t, rd, std::make_integer_sequence<int, kNB>{});
+        float *row = Xu + o + r + o * ldx;
+#pragma unroll
+        for (int s = 0; s < 16; ++s) row[(q + 4 * s) * ldx] = v[s];
+    } else {
+        llnu_all(v, q, t, std::make_integer_sequence<int, kNB>{});
+        float *col = Xl + o + (o + r) * ldx;
+#pragma unroll
+        for (int s = 0; s < 16; ++s) col[q + 4 * s] = v[s];
+    }
+}
+
+__global__ void k_zero2(int64_t n, float *__restrict__ A, float *__restrict__ B, int64_t ld) {
+    const int64_t j = blockIdx.y;
+    float *P = blockIdx.z ? B : A;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        P[i + j * ld] = 0.0f;
+}
+
 __global__ void k_identity(int64_t n, float *__restrict__ A, int64_t lda) {
     const int64_t j = blockIdx.y;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -284,7 +328,21 @@ int sgemm(int64_t m, int64_t n, int64_t k, float alpha, const float *A, int64_t 
     if (m <= 0 || n <= 0) return OMX_OK;
     dim3 grid((unsigned)omx_cdiv(m, 64), (unsigned)omx_cdiv(n, 64));
     if (grid.y > 65535) return OMX_ERR_ARG;
-    hipLaunchKernelGGL(k_sgemm_nn, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    hipLaunchKernelGGL(k_sgemm_nn, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                       (int64_t)0, (int64_t)0, (int64_t)0);
+    OMX_LAUNCH_CHECK();
+    return OMX_OK;
+}
+
+// batch products C_p = beta C_p + alpha A_p B_p, X_p = X + p * sX
+int sgemm_batched(int64_t m, int64_t n, int64_t k, float alpha, const float *A, int64_t lda, int64_t sA,
+                  const float *B, int64_t ldb, int64_t sB, float beta, float *C, int64_t ldc, int64_t sC,
+                  int64_t batch, hipStream_t st) {
+    if (m <= 0 || n <= 0 || batch <= 0) return OMX_OK;
+    dim3 grid((unsigned)omx_cdiv(m, 64), (unsigned)omx_cdiv(n, 64), (unsigned)batch);
+    if (grid.y > 65535 || batch > 65535 || k % 16) return OMX_ERR_ARG;
+    hipLaunchKernelGGL(k_sgemm_nn, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA, sB,
+                       sC);
     OMX_LAUNCH_CHECK();
     return OMX_OK;
 }
@@ -391,6 +449,60 @@ int omx_trsm_f32(int variant, int64_t m, int64_t n, const float *T, int64_t ldt,
         return OMX_OK;
     }
     return OMX_ERR_ARG;
+}
+
+int omx_trtri_lu_f32(int64_t n, const float *D, int64_t ldd, float *Xu, float *Xl, int64_t ldx, float *work,
+                     void *stream) {
+    if (n <= 0 || n % kNB || n > 65535 || !D || !Xu || !Xl || !work || ldd < n || ldx < n) return OMX_ERR_ARG;
+    hipStream_t st = omx_stream(stream);
+    const int nblk = (int)(n / kNB);
+    hipLaunchKernelGGL(k_zero2, dim3((unsigned)omx_cdiv(n, 256), (unsigned)n, 2u), dim3(256), 0, st, n, Xu, Xl, ldx);
+    OMX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tri_inv64, dim3((unsigned)(2 * nblk)), dim3(256), 0, st, nblk, D, ldd, Xu, Xl, ldx);
+    OMX_LAUNCH_CHECK();
+    // Merge adjacent ranges of diagonal blocks level by level: ranges of s0
+    // blocks pair up (the last, odd one waits for a later level).  For the
+    // pair [R1 | R2] with inverses X1, X2 on the diagonal:
+    //   U: X12 = -X1 (U12 X2)      L: X21 = -X2 (L21 X1)
+    // with the inner product in work (n x n, leading dimension n, at the
+    // position of the block it becomes).  Pairs of equal sizes run batched.
+    std::vector<int64_t> sz(nblk, kNB);  // range sizes in rows, in order
+    while (sz.size() > 1) {
+        std::vector<int64_t> next;
+        int64_t pos = 0;
+        size_t i = 0;
+        while (i + 1 < sz.size()) {
+            // a batch: consecutive pairs with the same (s1, s2)
+            const int64_t s1 = sz[i], s2 = sz[i + 1];
+            int64_t cnt = 0;
+            while (i + 1 < sz.size() && sz[i] == s1 && sz[i + 1] == s2) {
+                ++cnt;
+                i += 2;
+            }
+            const int64_t stride = s1 + s2;  // along the diagonal, per pair
+            const int64_t a0 = pos, a1 = pos + s1;
+            int rc;
+            // U: work(a0.., a1..) = U12 X2;  Xu(a0.., a1..) = -X1 work
+            rc = sgemm_batched(s1, s2, s2, 1.0f, D + a0 + a1 * ldd, ldd, stride * (ldd + 1), Xu + a1 + a1 * ldx, ldx,
+                               stride * (ldx + 1), 0.0f, work + a0 + a1 * n, n, stride * (n + 1), cnt, st);
+            if (rc) return rc;
+            rc = sgemm_batched(s1, s2, s1, -1.0f, Xu + a0 + a0 * ldx, ldx, stride * (ldx + 1), work + a0 + a1 * n, n,
+                               stride * (n + 1), 0.0f, Xu + a0 + a1 * ldx, ldx, stride * (ldx + 1), cnt, st);
+            if (rc) return rc;
+            // L: work(a1.., a0..) = L21 X1;  Xl(a1.., a0..) = -X2 work
+            rc = sgemm_batched(s2, s1, s1, 1.0f, D + a1 + a0 * ldd, ldd, stride * (ldd + 1), Xl + a0 + a0 * ldx, ldx,
+                               stride * (ldx + 1), 0.0f, work + a1 + a0 * n, n, stride * (n + 1), cnt, st);
+            if (rc) return rc;
+            rc = sgemm_batched(s2, s1, s2, -1.0f, Xl + a1 + a1 * ldx, ldx, stride * (ldx + 1), work + a1 + a0 * n, n,
+                               stride * (n + 1), 0.0f, Xl + a1 + a0 * ldx, ldx, stride * (ldx + 1), cnt, st);
+            if (rc) return rc;
+            for (int64_t c = 0; c < cnt; ++c) next.push_back(stride);
+            pos += cnt * stride;
+        }
+        if (i < sz.size()) next.push_back(sz[i]);
+        sz.swap(next);
+    }
+    return OMX_OK;
 }
 
 int omx_trsm_identity_f32(int variant, int64_t n, const float *T, int64_t ldt, float *X, int64_t ldx,

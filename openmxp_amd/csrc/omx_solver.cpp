@@ -68,6 +68,7 @@ struct omx_solver {
     int scale16 = 0;                    // FP16 Schur panels with per-strip scales
     DevBuf Lse[8], Rse[8];              // their strip exponents, per (group buffer, window)
     DevBuf Wup, Wlo, Wup_h, Wlo_h;      // -alt inverses (FP32, packed FP16)
+    DevBuf Wtmp;                        // their b x b scratch
     DevBuf info;
     // learned schedule: MLP parameters, per-step features, forward scratch
     DevBuf pol, feat, mlp_h, zdev;
@@ -242,6 +243,7 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
     s->last_feat.assign((size_t)nb * OMX_MLP_NFEATURES, 0.0f);
     if ((rc = dalloc(s->Tp, omx_panel_bytes(rows, b, 0)))) return fail(rc);
     if ((rc = dalloc(s->Wup, (size_t)b * b * 4)) || (rc = dalloc(s->Wlo, (size_t)b * b * 4)) ||
+        (rc = dalloc(s->Wtmp, (size_t)b * b * 4)) ||
         (rc = dalloc(s->Wup_h, omx_panel_bytes(b, b, 0))) || (rc = dalloc(s->Wlo_h, omx_panel_bytes(b, b, 0))))
         return fail(rc);
     if ((rc = dalloc(s->info, (size_t)nb * sizeof(int) + 16))) return fail(rc);
@@ -263,7 +265,7 @@ int omx_solver_destroy(omx_solver *s) {
             if (e) (void)hipEventDestroy(e);
     DevBuf *bufs[] = {&s->Lp[0], &s->Lp[1], &s->Rp[0], &s->Rp[1], &s->Tp, &s->L8[0], &s->L8[1], &s->R8[0],
                       &s->R8[1], &s->Le[0], &s->Le[1], &s->Re[0], &s->Re[1], &s->L8max, &s->R8max,
-                      &s->Wup, &s->Wlo, &s->Wup_h, &s->Wlo_h, &s->info, &s->r,
+                      &s->Wup, &s->Wlo, &s->Wup_h, &s->Wlo_h, &s->Wtmp, &s->info, &s->r,
                       &s->w, &s->gemv_ws, &s->trsv_ws, &s->nrm, &s->V, &s->hv, &s->dots,
                       &s->pol, &s->feat, &s->mlp_h, &s->zdev};
     for (DevBuf *d : bufs) dfree(*d);
@@ -468,12 +470,12 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
                 RC(omx_trsm_f32(OMX_TRSM_RUNN, rest, b, Akk, lda, Acol, lda, cs));
                 RC(omx_trsm_f32(OMX_TRSM_LLNU, b, rest, Akk, lda, Arow, lda, cs));
             } else {
-                RC(omx_trsm_identity_f32(OMX_TRSM_RUNN, b, Akk, lda, Wup, b, cs));
+                // U^-1 and L^-1 of the factored diagonal block
+                RC(omx_trtri_lu_f32(b, Akk, lda, Wup, Wlo, b, static_cast<float *>(s->Wtmp.p), cs));
                 // Y(j, l) = W(l, j), upper part only for -alt 2
                 RC(omx_pack_f16(b, b, Wup, b, 1, s->alt == 2 ? OMX_MASK_K_LE_R : OMX_MASK_NONE, Wup_h, nullptr, cs));
                 RC(omx_pack_f16(rest, b, Acol, 1, lda, OMX_MASK_NONE, Tp, nullptr, cs));
                 RC(omx_gemm_f16_packed(rest, b, b, 1.0f, Tp, 0, Wup_h, 0, 0.0f, Acol, lda, cs));
-                RC(omx_trsm_identity_f32(OMX_TRSM_RLNU, b, Akk, lda, Wlo, b, cs));
                 // X(i, l) = W(i, l), strict lower + unit diagonal for -alt 2
                 RC(omx_pack_f16(b, b, Wlo, 1, b, s->alt == 2 ? OMX_MASK_R_GT_K_UNIT : OMX_MASK_NONE, Wlo_h, nullptr,
                                 cs));

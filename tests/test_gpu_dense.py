@@ -151,3 +151,43 @@ def test_trsm_identity_argument_errors():
     assert L.omx_trsm_identity_f32(1, 128, ptr(t), 128, ptr(x), 128, None) < 0   # LLNU not offered
     assert L.omx_trsm_identity_f32(0, 100, ptr(t), 128, ptr(x), 128, None) < 0   # n % 64
     assert L.omx_trsm_identity_f32(0, 128, ptr(t), 64, ptr(x), 128, None) < 0    # ldt < n
+
+
+@pytest.mark.parametrize("n", [64, 320, 640, 2560])
+def test_trtri_lu_inverts_both_triangles(n):
+    """U^-1 and L^-1 of a factored block by diagonal-block substitution and
+    level-by-level merges (n / 64 = 1, 5, 10, 40 blocks: odd counts leave a
+    range for a later level): against FP64 inverses of the same FP32
+    triangles, the other triangle exactly zero, L^-1's diagonal exactly 1."""
+    torch = torch_cuda()
+    L = _L()
+    rng = np.random.default_rng(n)
+    Lt = np.tril(rng.uniform(-1, 1, (n, n)) / n, -1) + np.eye(n)
+    Ut = np.triu(rng.uniform(-1, 1, (n, n)) / 4, 1)
+    Ut += np.diag(1.0 + np.abs(Ut).sum(axis=1))
+    D = (np.tril(Lt, -1) + Ut).astype(np.float32)
+    Dd, ldd = _fdev(D, n + 16)
+    Xu = empty(n * n, torch.float32)
+    Xl = empty(n * n, torch.float32)
+    W = empty(n * n, torch.float32)
+    assert L.omx_trtri_lu_f32(n, ptr(Dd), ldd, ptr(Xu), ptr(Xl), n, ptr(W), None) == 0
+    xu = colmajor_to_numpy(Xu, n, n).astype(np.float64)
+    xl = colmajor_to_numpy(Xl, n, n).astype(np.float64)
+    U32 = np.triu(D).astype(np.float64)
+    L32 = np.tril(D, -1).astype(np.float64) + np.eye(n)
+    iu = np.linalg.inv(U32)
+    il = np.linalg.inv(L32)
+    print(f"  n={n}: |Xu - U^-1| {np.abs(xu - iu).max():.2e}, |Xl - L^-1| {np.abs(xl - il).max():.2e}", flush=True)
+    assert np.abs(xu - iu).max() <= 1e-5 * np.abs(iu).max()
+    assert np.abs(xl - il).max() <= 1e-5 * np.abs(il).max()
+    assert not np.any(np.tril(xu, -1)) and not np.any(np.triu(xl, 1))
+    assert np.all(np.diag(xl) == 1.0)
+
+
+def test_trtri_lu_argument_errors():
+    torch = torch_cuda()
+    L = _L()
+    a = empty(128 * 128, torch.float32)
+    assert L.omx_trtri_lu_f32(100, ptr(a), 128, ptr(a), ptr(a), 128, ptr(a), None) < 0   # n % 64
+    assert L.omx_trtri_lu_f32(128, ptr(a), 64, ptr(a), ptr(a), 128, ptr(a), None) < 0    # ldd < n
+    assert L.omx_trtri_lu_f32(128, ptr(a), 128, ptr(a), ptr(a), 128, None, None) < 0     # no work
