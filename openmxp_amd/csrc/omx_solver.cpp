@@ -13,14 +13,18 @@
 // G times (its traffic, not the MFMA work, bounds the update's epilogue).
 // The panels of a group share one packed buffer (panel G g + j in K columns
 // [j b, (j+1) b)).  Group g, with panels G g .. G g + G - 1 factored:
-//   bulk stream  A[G(g+2):, G(g+2):] -= [L_Gg .. L_Gg+G-1] [R_Gg .. R_Gg+G-1]^T
+//   bulk stream  block row/col t -= group g for each block t of the next
+//                group (the look-ahead products), then
+//                A[G(g+2):, G(g+2):] -= [L_Gg .. L_Gg+G-1] [R_Gg .. R_Gg+G-1]^T
 //   crit stream  for each block t of the next group, in order:
-//                block row/col t -= group g, then -= the next group's
-//                panels before t; factor panel t
-// The crit stream waits for the bulk product of group g-1 (it covered the
-// next group's blocks), the bulk product of group g for group g's panels; so
-// the factorisation of the next group's panels overlaps the trailing update
-// of the current group.  Runs of a group's panels packed in the same
+//                block row/col t -= the next group's panels before t;
+//                factor panel t
+// The crit stream waits for the look-ahead products of group g (queued on
+// the bulk stream after the trailing update of group g-1, which covered the
+// next group's blocks), the bulk stream for group g's panels; so the
+// factorisation of the next group's panels overlaps the trailing update of
+// the current group, and the look-ahead products run beside the
+// factorisation chain rather than beside the trailing update.  Runs of a group's panels packed in the same
 // precision are applied as one product; panels in scaled FP16 (per-panel
 // scales) one by one.
 #include <hip/hip_runtime.h>
@@ -60,7 +64,7 @@ struct omx_solver {
     int alt = 1;
     int fp8 = 0;
     hipStream_t crit = nullptr, bulk = nullptr;
-    std::vector<hipEvent_t> ev_panel, ev_bulk, ev_t0, ev_t1;
+    std::vector<hipEvent_t> ev_panel, ev_t0, ev_t1;
     int64_t G = 2;                      // panels per trailing update (group)
     DevBuf Lp[2], Rp[2], Tp;            // FP16 group buffers (K = G b) and the -alt temporary
     DevBuf L8[2], R8[2], Le[2], Re[2];  // FP8 group buffers and their exponents
@@ -74,6 +78,7 @@ struct omx_solver {
     DevBuf pol, feat, mlp_h, zdev;
     int has_policy = 0;
     hipEvent_t ev_dec = nullptr;
+    hipEvent_t ev_la = nullptr;        // look-ahead products of the current group done
     float *zhost = nullptr;            // pinned: the step's 3 outputs
     std::vector<int> last_sched;       // per step: 0 FP16, 1 + headroom FP8
     std::vector<int> prec;             // the current factorisation's choices
@@ -199,13 +204,12 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
     if (hipStreamCreateWithFlags(&s->crit, hipStreamNonBlocking) != hipSuccess) return fail(OMX_ERR_HIP);
     if (hipStreamCreateWithFlags(&s->bulk, hipStreamNonBlocking) != hipSuccess) return fail(OMX_ERR_HIP);
     const int64_t nb = s->nb;
+    if (hipEventCreateWithFlags(&s->ev_la, hipEventDisableTiming) != hipSuccess) return fail(OMX_ERR_HIP);
     s->ev_panel.resize(nb);
-    s->ev_bulk.resize(nb);
     s->ev_t0.resize(nb);
     s->ev_t1.resize(nb);
     for (int64_t k = 0; k < nb; ++k) {
         if (hipEventCreateWithFlags(&s->ev_panel[k], hipEventDisableTiming) != hipSuccess) return fail(OMX_ERR_HIP);
-        if (hipEventCreateWithFlags(&s->ev_bulk[k], hipEventDisableTiming) != hipSuccess) return fail(OMX_ERR_HIP);
         if (hipEventCreate(&s->ev_t0[k]) != hipSuccess) return fail(OMX_ERR_HIP);
         if (hipEventCreate(&s->ev_t1[k]) != hipSuccess) return fail(OMX_ERR_HIP);
     }
@@ -260,7 +264,7 @@ int omx_solver_destroy(omx_solver *s) {
     if (!s) return OMX_OK;
     if (s->crit) (void)hipStreamSynchronize(s->crit);
     if (s->bulk) (void)hipStreamSynchronize(s->bulk);
-    for (auto *v : {&s->ev_panel, &s->ev_bulk, &s->ev_t0, &s->ev_t1})
+    for (auto *v : {&s->ev_panel, &s->ev_t0, &s->ev_t1})
         for (auto e : *v)
             if (e) (void)hipEventDestroy(e);
     DevBuf *bufs[] = {&s->Lp[0], &s->Lp[1], &s->Rp[0], &s->Rp[1], &s->Tp, &s->L8[0], &s->L8[1], &s->R8[0],
@@ -274,6 +278,7 @@ int omx_solver_destroy(omx_solver *s) {
         dfree(s->Rse[i]);
     }
     if (s->ev_dec) (void)hipEventDestroy(s->ev_dec);
+    if (s->ev_la) (void)hipEventDestroy(s->ev_la);
     if (s->zhost) (void)hipHostFree(s->zhost);
     if (s->crit) (void)hipStreamDestroy(s->crit);
     if (s->bulk) (void)hipStreamDestroy(s->bulk);
@@ -507,25 +512,25 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
     OMX_HIP_CHECK(hipEventRecord(s->ev_panel[0], cs));
     for (int64_t g = 0; G * (g + 1) < nb; ++g) {
         const int64_t i0 = G * g, i1 = std::min(G * g + G - 1, nb - 1);
-        // ---- bulk: the group's trailing update beyond the next group ----
+        const int64_t h0 = G * (g + 1), h1 = std::min(h0 + G - 1, nb - 1);
+        // ---- bulk: the next group's blocks, then the trailing update ----
+        OMX_HIP_CHECK(hipStreamWaitEvent(bs, s->ev_panel[g], 0));
+        for (int64_t t = h0; t <= h1; ++t) RC(update_block(s, t, i0, i1, A, lda, bs));
+        OMX_HIP_CHECK(hipEventRecord(s->ev_la, bs));
         const int64_t f = G * (g + 2);
         if (f < nb) {
             const int64_t m = n - f * b;
-            OMX_HIP_CHECK(hipStreamWaitEvent(bs, s->ev_panel[g], 0));
             OMX_HIP_CHECK(hipEventRecord(s->ev_t0[s->bulk_count], bs));
             RC(apply_panels(s, i0, i1, f * b, f * b, m, m, A, lda, bs));
             OMX_HIP_CHECK(hipEventRecord(s->ev_t1[s->bulk_count], bs));
             s->bulk_flops += 2.0 * (double)m * m * (double)((i1 - i0 + 1) * b);
             ++s->bulk_count;
         }
-        OMX_HIP_CHECK(hipEventRecord(s->ev_bulk[g], bs));
         // ---- crit: the next group's panels ----
-        // its blocks hold the earlier groups' updates once the previous bulk
-        // product is done
-        if (g >= 1) OMX_HIP_CHECK(hipStreamWaitEvent(cs, s->ev_bulk[g - 1], 0));
-        const int64_t h0 = G * (g + 1), h1 = std::min(h0 + G - 1, nb - 1);
+        // its blocks hold all earlier groups' updates once the look-ahead
+        // products are done
+        OMX_HIP_CHECK(hipStreamWaitEvent(cs, s->ev_la, 0));
         for (int64_t t = h0; t <= h1; ++t) {
-            RC(update_block(s, t, i0, i1, A, lda, cs));
             if (t > h0) RC(update_block(s, t, h0, t - 1, A, lda, cs));
             RC(factor(t));
         }
