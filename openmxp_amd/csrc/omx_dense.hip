@@ -84,34 +84,34 @@ k_sgemm_nn(int64_t M, int64_t N, int64_t K, float alpha, const float *__restrict
 
 // ---------------------------------------------------------------------------
 // 64 x 64 diagonal piece of the LU: unblocked right-looking LU in one wave.
-// Lane i holds row i in registers.  Step k: lane k publishes its row
-// through LDS, every lane below scales its column-k entry by 1/u_kk (the
-// multiplier) and subtracts multiplier x row k from the rest of its row.
+// Lane i holds row i in registers.  Step k: every lane reads row k from
+// lane k (v_readlane into scalar registers, no LDS round trip); the lanes
+// below scale their column-k entry by 1/u_kk (the multiplier) and subtract
+// multiplier x row k from the rest of their row.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ float lane_bcast(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+
 __global__ void __launch_bounds__(kNB)
 k_getrf_diag64(float *__restrict__ A, int64_t lda, int64_t row0, int *__restrict__ info) {
-    __shared__ float urow[kNB];
     const int i = threadIdx.x;
     float r[kNB];
 #pragma unroll
     for (int j = 0; j < kNB; ++j) r[j] = A[i + (int64_t)j * lda];
 #pragma unroll
     for (int k = 0; k < kNB; ++k) {
-        if (i == k) {
-#pragma unroll
-            for (int j = k; j < kNB; ++j) urow[j] = r[j];
-        }
-        __syncthreads();
-        const float ukk = urow[k];
+        const float ukk = lane_bcast(r[k], k);
         if (i == 0 && ukk == 0.0f && info) atomicMin(info, (int)(row0 + k + 1));
         const float inv = 1.0f / ukk;
-        if (i > k) {
-            const float l = r[k] * inv;
-            r[k] = l;
+        const float l = r[k] * inv;
+        const bool below = i > k;
+        if (below) r[k] = l;
 #pragma unroll
-            for (int j = k + 1; j < kNB; ++j) r[j] = __builtin_fmaf(-l, urow[j], r[j]);
+        for (int j = k + 1; j < kNB; ++j) {
+            const float u = lane_bcast(r[j], k);
+            r[j] = below ? __builtin_fmaf(-l, u, r[j]) : r[j];
         }
-        __syncthreads();
     }
 #pragma unroll
     for (int j = 0; j < kNB; ++j) A[i + (int64_t)j * lda] = r[j];
