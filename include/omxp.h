@@ -399,6 +399,15 @@ int omx_mlp_backward_f32(int64_t batch, const float *params, const float *X,
                          float *grad, float *loss, void *work,
                          size_t work_bytes, void *stream);
 
+/* One SGD step on params: with momentum mu > 0, v <- mu v + g and
+ * p <- p - lr v (vel: OMX_MLP_NPARAMS floats, zero at the start); with
+ * mu = 0, p <- p - lr g (vel may be NULL).  Each operation rounded in FP32
+ * as written (no fused multiply-add).  The plain optimiser of the
+ * scheduler's training; omx_mlp_adam_f32 is the one the committed training
+ * uses. */
+int omx_mlp_sgd_f32(int64_t nparams, float *params, const float *grad, float *vel,
+                    float lr, float momentum, void *stream);
+
 /* One Adam step (bias-corrected, step >= 1) on params with moments mom, vel. */
 int omx_mlp_adam_f32(int64_t nparams, float *params, const float *grad,
                      float *mom, float *vel, float lr, float beta1,

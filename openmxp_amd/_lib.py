@@ -68,6 +68,7 @@ SIGNATURES = {
     "omx_mlp_forward_f32": (i32, [i64, vp, vp, vp, vp, vp, vp]),
     "omx_mlp_backward_workspace_bytes": (sz, [i64]),
     "omx_mlp_backward_f32": (i32, [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, vp]),
+    "omx_mlp_sgd_f32": (i32, [i64, vp, vp, vp, flt, flt, vp]),
     "omx_mlp_adam_f32": (i32, [i64, vp, vp, vp, vp, flt, flt, flt, flt, i32, vp]),
     "omx_sched_features_f32": (i32, [i64, vp, i64, vp, vp, i64, i64, i64, i64, i64, i32, vp, vp]),
     "omx_solver_create": (i32, [i64, i64, i32, i32, C.POINTER(vp)]),

@@ -165,6 +165,18 @@ __global__ void k_mlp_adam(int64_t np, float *__restrict__ params, const float *
     params[q] -= lr * (mq / c1) / (sqrtf(vq / c2) + eps);
 }
 
+__global__ void k_mlp_sgd(int64_t np, float *__restrict__ params, const float *__restrict__ grad,
+                          float *__restrict__ vel, float lr, float mu) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    float g = grad[q];
+    if (vel) {
+        g = mu * vel[q] + g;
+        vel[q] = g;
+    }
+    params[q] -= lr * g;
+}
+
 // Statistics of log2 of the non-zero tile maxima of a panel: max, min,
 // mean, standard deviation (thread-strided partial sums, then a fixed-order
 // tree in LDS).
@@ -317,6 +329,17 @@ int omx_mlp_adam_f32(int64_t nparams, float *params, const float *grad, float *m
     const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
     hipLaunchKernelGGL(k_mlp_adam, dim3((unsigned)omx_cdiv(nparams, 256)), dim3(256), 0, omx_stream(stream),
                        nparams, params, grad, mom, vel, lr, beta1, beta2, eps, c1, c2);
+    OMX_LAUNCH_CHECK();
+    return OMX_OK;
+}
+
+int omx_mlp_sgd_f32(int64_t nparams, float *params, const float *grad, float *vel, float lr, float momentum,
+                    void *stream) {
+    if (nparams <= 0 || !params || !grad || !(lr > 0) || !(momentum >= 0) || !(momentum < 1) ||
+        (momentum > 0 && !vel))
+        return OMX_ERR_ARG;
+    hipLaunchKernelGGL(k_mlp_sgd, dim3((unsigned)omx_cdiv(nparams, 256)), dim3(256), 0, omx_stream(stream),
+                       nparams, params, grad, momentum > 0 ? vel : nullptr, lr, momentum);
     OMX_LAUNCH_CHECK();
     return OMX_OK;
 }

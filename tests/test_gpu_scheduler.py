@@ -77,6 +77,34 @@ def test_adam_step_matches_numpy():
     assert np.allclose(host(M), m1, rtol=1e-5, atol=1e-7) and np.allclose(host(V), v1, rtol=1e-5, atol=1e-9)
 
 
+@pytest.mark.parametrize("mu", [0.0, 0.9])
+def test_sgd_step_matches_numpy(mu):
+    """Two SGD steps, FP32 as written (v = mu v + g; p -= lr v): bit-exact
+    against numpy float32."""
+    torch = torch_cuda()
+    L = _L()
+    rng = np.random.default_rng(7)
+    n = 5443
+    p = rng.standard_normal(n).astype(np.float32)
+    g1 = rng.standard_normal(n).astype(np.float32)
+    g2 = rng.standard_normal(n).astype(np.float32)
+    d = lambda a: torch.from_numpy(a.copy()).cuda()
+    P, V = d(p), d(np.zeros(n, np.float32))
+    f = np.float32
+    lr = f(0.05)
+    want, v = p.copy(), np.zeros(n, np.float32)
+    for g in (g1, g2):
+        G = d(g)
+        assert L.omx_mlp_sgd_f32(n, ptr(P), ptr(G), ptr(V) if mu else None, 0.05, mu, None) == 0
+        if mu:
+            v = f(mu) * v + g
+            want = want - lr * v
+        else:
+            want = want - lr * g
+    assert np.array_equal(host(P), want)
+    assert L.omx_mlp_sgd_f32(n, ptr(P), ptr(G), None, 0.05, 0.9, None) < 0   # momentum without a buffer
+
+
 def numpy_features(lmax, rmax, udiag, step, nb, n, b, prev):
     def stats(v):
         v = v[(v > 0) & np.isfinite(v)]
