@@ -214,12 +214,12 @@ def test_scaled_fp16_lu_matches_plain():
     assert berr[1] <= 1.05 * berr[0]
 
 
-@pytest.mark.parametrize("n,alt", [(2560, 1), (3584, 0), (3584, 1)])
+@pytest.mark.parametrize("n,alt", [(768, 1), (1280, 2), (2560, 1), (3584, 0), (3584, 1)])
 def test_lu_group4_matches_oracle(orc, n, alt):
     """OMX_SOLVER_GROUP4: the trailing matrix takes one product per four
     panels (depth 4b); the factors agree with the oracle's reference-order
-    LU to the FP16-update tolerance, also with a last group of two panels
-    (n / b = 10, 14)."""
+    LU to the FP16-update tolerance, also with fewer panels than one group
+    (n / b = 3), a last group of one or two panels (5, 10, 14)."""
     torch = torch_cuda()
     b, seed = 256, 42
     print(f"lu group4 n={n} b={b} alt={alt}", flush=True)
