@@ -24,7 +24,9 @@
  *                                                                    numpy float16, exhaustive over f16 values
  *   orc_f32_to_e4m3/e4m3_to_  new (FP8 e4m3fn, RNE, saturating)      torch.float8_e4m3fn conversion
  *   orc_getrf_nopiv           sgetrf_nopiv.cpp:27-61                 compiled sgetrf_nopiv_naive (_ref) bit-exact,
- *                                                                    scipy LU residual
+ *                                                                    compiled sgetrf_nopiv_tuned within 1e-4
+ *                                                                    (other blocking), LAPACK getrf on
+ *                                                                    column-dominant matrices, zero pivot
  *   orc_trsm_*                getrf_nopiv.hpp:7-24 (BLAS strsm)      scipy.linalg.solve_triangular
  *   orc_trtri_*               rocsolver_strtri call sites             numpy.linalg.inv, T*inv(T) = I
  *                             (panel_trf_gdirect.hpp:813-829),

@@ -111,6 +111,7 @@ def ref_lib():
             "ref_f16_to_f32": (flt, [C.c_uint16]),
             "ref_shgemm": (None, [C.c_int, C.c_int, C.c_int, flt, vp, C.c_int, vp, C.c_int, flt, vp, C.c_int]),
             "ref_sgetrf_nopiv_tuned": (None, [C.c_int, vp, C.c_size_t]),
+            "ref_sgetrf_nopiv_naive": (None, [C.c_int, C.c_int, vp, C.c_size_t]),
         }
         for name, (res, args) in sig.items():
             f = getattr(R, name)

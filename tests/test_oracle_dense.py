@@ -24,6 +24,19 @@ def test_getrf_matches_reference_tuned_lu(orc, ref):
     assert np.max(np.abs(LU - R) / (np.abs(R) + 1e-3)) < 1e-4
 
 
+@pytest.mark.parametrize("n", [64, 300])
+def test_getrf_matches_reference_naive_lu_bit_exact(orc, ref, n):
+    """The oracle follows sgetrf_nopiv_naive's operation order (scale the
+    column by the reciprocal pivot, then column-by-column updates, no fused
+    multiply-add): identical bits."""
+    A = _dd_matrix(n, 3)
+    LU, info = orc.getrf_nopiv(A)
+    assert info == 0
+    R = np.array(A, order="F", copy=True)
+    ref.ref_sgetrf_nopiv_naive(n, n, R.ctypes.data, n)
+    assert np.array_equal(LU, R)
+
+
 def test_getrf_matches_lapack_on_dominant_matrix(orc):
     """Partial pivoting never swaps rows of a column-dominant matrix, so LAPACK's
     getrf is the no-pivot LU; compare in FP64."""
