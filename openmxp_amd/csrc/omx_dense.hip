@@ -9,6 +9,7 @@
 // multiplication by the reciprocal, as sgetrf_nopiv_naive does
 // (sgetrf_nopiv.cpp:27-61) and as reference BLAS strsm does for a non-unit
 // diagonal.
+#include <algorithm>
 #include <utility>
 #include <vector>
 
@@ -17,6 +18,17 @@
 namespace {
 
 constexpr int kNB = 64;
+#define RC(x)                  \
+    do {                       \
+        const int rc_ = (x);   \
+        if (rc_) return rc_;   \
+    } while (0)
+
+// The triangular solves (omx_trsm_f32) go 64 columns at a time inside
+// blocks of kOB: the 64-wide steps update only the rest of their block, and
+// each block updates everything beyond it in one product of depth kOB, so
+// the large products read and write their C a quarter as often.
+constexpr int64_t kOB = 256;
 
 // ---------------------------------------------------------------------------
 // FP32 GEMM, C = beta*C + alpha * A * B (all column-major, no transposes).
@@ -78,6 +90,163 @@ k_sgemm_nn(int64_t M, int64_t N, int64_t K, float alpha, const float *__restrict
             if (gj >= N) continue;
             float *c = C + gi + gj * ldc;
             *c = (beta == 0.0f) ? alpha * acc[p][q] : beta * *c + alpha * acc[p][q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The same product on the matrix cores (v_mfma_f32_32x32x2_f32), for the
+// large updates: a 128 x 128 tile per workgroup of four waves (a 64 x 64
+// quadrant each, 2 x 2 MFMA blocks of 32 x 32), k staged through LDS up to
+// 64 at a time, the next stage's global loads in flight during the current
+// stage's MFMAs.  The MFMA's first operand takes B (lane l: column l % 32),
+// the second A (lane l: row l % 32), so every accumulator register of lane l
+// belongs to row l % 32 of its block and a half-wave's C access is 32
+// consecutive rows of one column (one 128-byte line).  In a stage of kc
+// columns, lanes 0-31 carry k in [0, kc/2) and lanes 32-63 k in [kc/2, kc);
+// the MFMA adds the two halves.  Needs A and B 16-byte aligned with lda and
+// ldb multiples of 4, and k % 16 == 0 (so kc is a multiple of 16).
+// ---------------------------------------------------------------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kMT = 128;        // tile rows and columns
+constexpr int kMK = 64;         // k per LDS stage
+constexpr int kBLd = kMK + 4;   // row stride of Bs: conflict-free 16-byte reads
+
+template <bool kOneStage>
+__global__ void __launch_bounds__(256)
+k_sgemm_mfma(int64_t M, int64_t N, int64_t K, float alpha, const float *__restrict__ A, int64_t lda,
+             const float *__restrict__ B, int64_t ldb, float beta, float *__restrict__ C, int64_t ldc,
+             int64_t sA, int64_t sB, int64_t sC) {
+    __shared__ float As[kMK][kMT];   // As[k][i] = A(i0 + i, k0 + k)
+    __shared__ float Bs[kMT][kBLd];  // Bs[j][k] = B(k0 + k, j0 + j)
+    A += (int64_t)blockIdx.z * sA;
+    B += (int64_t)blockIdx.z * sB;
+    C += (int64_t)blockIdx.z * sC;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, h = lane >> 5;
+    const int64_t i0 = (int64_t)blockIdx.x * kMT, j0 = (int64_t)blockIdx.y * kMT;
+    const int wi = (wave & 1) * 64, wj = (wave >> 1) * 64;
+    // global -> LDS: A rows ar .. ar+3 at stage columns ak + 8 s; B column
+    // bj + 16 s, stage rows bk .. bk+3.  Rows (columns) past M (N) are
+    // clamped: their products land only in results that are not stored.
+    const int ar = 4 * (tid & 31), ak = tid >> 5;
+    const int bk = 4 * (tid & 15), bj = tid >> 4;
+    const int64_t arow = i0 + ar;
+    const bool afull = arow + 3 < M;
+    int64_t ar1 = arow + 1, ar2 = arow + 2, ar3 = arow + 3;
+    const int64_t arow0 = arow < M ? arow : M - 1;
+    if (ar1 >= M) ar1 = M - 1;
+    if (ar2 >= M) ar2 = M - 1;
+    if (ar3 >= M) ar3 = M - 1;
+    const float *Bcol[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        int64_t j = j0 + bj + 16 * s;
+        if (j >= N) j = N - 1;
+        Bcol[s] = B + j * ldb;
+    }
+    f32x4 ra[8], rb[8];
+    // stage k0 .. k0 + kc into ra / rb.  Every load is unconditional: stage
+    // columns (rows) at or past kc read column (row) kc - 1 (kc - 4) instead,
+    // and are not stored to LDS.
+#define OMX_SGEMM_GLOAD(k0, kc)                                                                            \
+    {                                                                                                      \
+        _Pragma("unroll") for (int s = 0; s < 8; ++s) {                                                    \
+            const int kk = ak + 8 * s < (kc) ? ak + 8 * s : (kc) - 1;                                      \
+            const float *p = A + ((k0) + kk) * lda;                                                        \
+            ra[s] = afull ? *reinterpret_cast<const f32x4 *>(p + arow)                                    \
+                          : f32x4{p[arow0], p[ar1], p[ar2], p[ar3]};                                 \
+        }                                                                                                  \
+        const int kb4 = bk < (kc) ? bk : (kc) - 4;                                                         \
+        _Pragma("unroll") for (int s = 0; s < 8; ++s)                                                      \
+            rb[s] = *reinterpret_cast<const f32x4 *>(Bcol[s] + (k0) + kb4);                               \
+    }
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
+    int64_t k0 = 0;
+    int kc = (int)(K < kMK ? K : kMK);
+    OMX_SGEMM_GLOAD(0, kc);
+    // kOneStage (K <= 64): C of the wave's quadrant is read now, with the
+    // operands, so that its latency overlaps theirs and the MFMAs (rows /
+    // columns past the edge read a clamped entry and are not stored);
+    // cv[a][b][v] sits where acc[a][b][v] does (see below).  With several
+    // stages the staging registers are live through the loop and C is read
+    // in the epilogue.
+    float cv[2][2][16];
+    if (kOneStage && beta != 0.0f) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            int64_t i = i0 + wi + 32 * a + l32;
+            if (i >= M) i = M - 1;
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    int64_t j = j0 + wj + 32 * b + 4 * h + 8 * (v >> 2) + (v & 3);
+                    if (j >= N) j = N - 1;
+                    cv[a][b][v] = C[i + j * ldc];
+                }
+        }
+    }
+    for (;;) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            if (ak + 8 * s < kc) *reinterpret_cast<f32x4 *>(&As[ak + 8 * s][ar]) = ra[s];
+        if (bk < kc) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) *reinterpret_cast<f32x4 *>(&Bs[bj + 16 * s][bk]) = rb[s];
+        }
+        __syncthreads();
+        const int64_t kn = k0 + kc;
+        const int kcn = (!kOneStage && kn < K) ? (int)(K - kn < kMK ? K - kn : kMK) : 0;
+        if (kcn) OMX_SGEMM_GLOAD(kn, kcn);
+        const int half = kc >> 1, kb = h * half;
+        for (int s = 0; s < half; s += 4) {
+            const f32x4 y0 = *reinterpret_cast<const f32x4 *>(&Bs[wj + l32][kb + s]);
+            const f32x4 y1 = *reinterpret_cast<const f32x4 *>(&Bs[wj + 32 + l32][kb + s]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float x0 = As[kb + s + u][wi + l32];
+                const float x1 = As[kb + s + u][wi + 32 + l32];
+                acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(y0[u], x0, acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(y1[u], x0, acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(y0[u], x1, acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(y1[u], x1, acc[1][1], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        if (!kcn) break;
+        k0 = kn;
+        kc = kcn;
+    }
+#undef OMX_SGEMM_GLOAD
+    // acc[a][b][v]: row i0 + wi + 32 a + l32, column j0 + wj + 32 b + 8 (v / 4) + 4 h + v % 4
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int64_t i = i0 + wi + 32 * a + l32;
+        if (i >= M) continue;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int64_t jb = j0 + wj + 32 * b + 4 * h;
+            if (!kOneStage && beta != 0.0f) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int64_t j = jb + 8 * (v >> 2) + (v & 3);
+                    cv[a][b][v] = j < N ? C[i + j * ldc] : 0.0f;
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int64_t j = jb + 8 * (v >> 2) + (v & 3);
+                if (j < N)
+                    C[i + j * ldc] = (beta == 0.0f) ? alpha * acc[a][b][v] : beta * cv[a][b][v] + alpha * acc[a][b][v];
+            }
         }
     }
 }
@@ -323,9 +492,42 @@ __global__ void k_copy_block(int64_t m, const float *__restrict__ A, int64_t lda
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
+// The MFMA kernel takes products with at least this many C entries per
+// launch (2^18 = 512 x 512; smaller ones are latency-bound and the 64 x 64
+// VALU tiles spread them over more CUs) when its alignment conditions hold.
+#ifndef OMX_SGEMM_MFMA_MIN
+#define OMX_SGEMM_MFMA_MIN (int64_t{1} << 18)
+#endif
+
+bool use_mfma(int64_t m, int64_t n, int64_t k, int64_t batch, const float *A, int64_t lda, const float *B,
+              int64_t ldb) {
+    const bool aligned = (reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0 &&
+                         (lda & 3) == 0 && (ldb & 3) == 0 && k >= 16 && k % 16 == 0;
+    return aligned && m * n * batch >= OMX_SGEMM_MFMA_MIN;
+}
+
+int sgemm_mfma(int64_t m, int64_t n, int64_t k, float alpha, const float *A, int64_t lda, int64_t sA, const float *B,
+               int64_t ldb, int64_t sB, float beta, float *C, int64_t ldc, int64_t sC, int64_t batch, hipStream_t st) {
+    const dim3 grid((unsigned)omx_cdiv(m, kMT), (unsigned)omx_cdiv(n, kMT), (unsigned)batch);
+    if (k <= kMK)
+        hipLaunchKernelGGL(k_sgemm_mfma<true>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA,
+                           sB, sC);
+    else
+        hipLaunchKernelGGL(k_sgemm_mfma<false>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA,
+                           sB, sC);
+    OMX_LAUNCH_CHECK();
+    return OMX_OK;
+}
+
+// msel: the row count the kernel is chosen for (default m).  A caller that
+// skips rows known to be zero passes the full count, so that the rows it
+// does compute round exactly as in the full product.
 int sgemm(int64_t m, int64_t n, int64_t k, float alpha, const float *A, int64_t lda, const float *B,
-          int64_t ldb, float beta, float *C, int64_t ldc, hipStream_t st) {
+          int64_t ldb, float beta, float *C, int64_t ldc, hipStream_t st, int64_t msel = -1) {
     if (m <= 0 || n <= 0) return OMX_OK;
+    if (use_mfma(msel < 0 ? m : msel, n, k, 1, A, lda, B, ldb)) {
+        return sgemm_mfma(m, n, k, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1, st);
+    }
     dim3 grid((unsigned)omx_cdiv(m, 64), (unsigned)omx_cdiv(n, 64));
     if (grid.y > 65535) return OMX_ERR_ARG;
     hipLaunchKernelGGL(k_sgemm_nn, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
@@ -341,6 +543,9 @@ int sgemm_batched(int64_t m, int64_t n, int64_t k, float alpha, const float *A, 
     if (m <= 0 || n <= 0 || batch <= 0) return OMX_OK;
     dim3 grid((unsigned)omx_cdiv(m, 64), (unsigned)omx_cdiv(n, 64), (unsigned)batch);
     if (grid.y > 65535 || batch > 65535 || k % 16) return OMX_ERR_ARG;
+    if (use_mfma(m, n, k, batch, A, lda, B, ldb) && (sA & 3) == 0 && (sB & 3) == 0) {
+        return sgemm_mfma(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, st);
+    }
     hipLaunchKernelGGL(k_sgemm_nn, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA, sB,
                        sC);
     OMX_LAUNCH_CHECK();
@@ -349,11 +554,11 @@ int sgemm_batched(int64_t m, int64_t n, int64_t k, float alpha, const float *A, 
 
 // sgemm with n split into chunks of at most 65535*64 columns
 int sgemm_big(int64_t m, int64_t n, int64_t k, float alpha, const float *A, int64_t lda, const float *B,
-              int64_t ldb, float beta, float *C, int64_t ldc, hipStream_t st) {
+              int64_t ldb, float beta, float *C, int64_t ldc, hipStream_t st, int64_t msel = -1) {
     const int64_t nmax = (int64_t)65535 * 64;
     for (int64_t j = 0; j < n; j += nmax) {
         const int64_t nn = (n - j < nmax) ? n - j : nmax;
-        int rc = sgemm(m, nn, k, alpha, A, lda, B + j * ldb, ldb, beta, C + j * ldc, ldc, st);
+        int rc = sgemm(m, nn, k, alpha, A, lda, B + j * ldb, ldb, beta, C + j * ldc, ldc, st, msel);
         if (rc) return rc;
     }
     return OMX_OK;
@@ -404,26 +609,31 @@ int omx_trsm_f32(int variant, int64_t m, int64_t n, const float *T, int64_t ldt,
         const unsigned g = (unsigned)omx_cdiv(m, 256);
         if (variant == OMX_TRSM_RUNN) {
             const unsigned g4 = (unsigned)omx_cdiv(m, 64);  // a lane quad per row
-            for (int64_t j = 0; j < n; j += kNB) {
-                hipLaunchKernelGGL(k_trsm_rows_runn64, dim3(g4), dim3(256), 0, st, m, T + j + j * ldt, ldt,
-                                   B + j * ldb, ldb);
-                OMX_LAUNCH_CHECK();
-                const int64_t rest = n - j - kNB;
-                if (rest > 0) {
-                    int rc = sgemm_big(m, rest, kNB, -1.0f, B + j * ldb, ldb, T + j + (j + kNB) * ldt, ldt,
-                                       1.0f, B + (j + kNB) * ldb, ldb, st);
-                    if (rc) return rc;
+            for (int64_t J0 = 0; J0 < n; J0 += kOB) {
+                const int64_t w = std::min(kOB, n - J0);
+                for (int64_t j = J0; j < J0 + w; j += kNB) {
+                    hipLaunchKernelGGL(k_trsm_rows_runn64, dim3(g4), dim3(256), 0, st, m, T + j + j * ldt, ldt,
+                                       B + j * ldb, ldb);
+                    OMX_LAUNCH_CHECK();
+                    const int64_t inner = J0 + w - j - kNB;
+                    if (inner > 0) RC(sgemm_big(m, inner, kNB, -1.0f, B + j * ldb, ldb, T + j + (j + kNB) * ldt, ldt, 1.0f,
+                                                B + (j + kNB) * ldb, ldb, st));
                 }
+                const int64_t rest = n - J0 - w;
+                if (rest > 0) RC(sgemm_big(m, rest, w, -1.0f, B + J0 * ldb, ldb, T + J0 + (J0 + w) * ldt, ldt, 1.0f,
+                                           B + (J0 + w) * ldb, ldb, st));
             }
         } else {
-            for (int64_t j = n - kNB; j >= 0; j -= kNB) {
-                hipLaunchKernelGGL(k_trsm_rows_rlnu64, dim3(g), dim3(256), 0, st, m, T + j + j * ldt, ldt,
-                                   B + j * ldb, ldb);
-                OMX_LAUNCH_CHECK();
-                if (j > 0) {
-                    int rc = sgemm_big(m, j, kNB, -1.0f, B + j * ldb, ldb, T + j, ldt, 1.0f, B, ldb, st);
-                    if (rc) return rc;
+            for (int64_t J1 = n; J1 > 0; J1 -= kOB) {
+                const int64_t J0 = std::max<int64_t>(0, J1 - kOB);
+                for (int64_t j = J1 - kNB; j >= J0; j -= kNB) {
+                    hipLaunchKernelGGL(k_trsm_rows_rlnu64, dim3(g), dim3(256), 0, st, m, T + j + j * ldt, ldt,
+                                       B + j * ldb, ldb);
+                    OMX_LAUNCH_CHECK();
+                    if (j > J0) RC(sgemm_big(m, j - J0, kNB, -1.0f, B + j * ldb, ldb, T + j + J0 * ldt, ldt, 1.0f,
+                                             B + J0 * ldb, ldb, st));
                 }
+                if (J0 > 0) RC(sgemm_big(m, J0, J1 - J0, -1.0f, B + J0 * ldb, ldb, T + J0, ldt, 1.0f, B, ldb, st));
             }
         }
         return OMX_OK;
@@ -432,19 +642,22 @@ int omx_trsm_f32(int variant, int64_t m, int64_t n, const float *T, int64_t ldt,
         // inverse of an m x m unit lower triangle times B (m x n), m % 64 == 0
         if (m % kNB || ldt < m) return OMX_ERR_ARG;
         const int64_t nmax = (int64_t)65535 * 64;  // a lane quad per column
-        for (int64_t i = 0; i < m; i += kNB) {
-            for (int64_t c = 0; c < n; c += nmax) {
-                const int64_t nn = (n - c < nmax) ? n - c : nmax;
-                hipLaunchKernelGGL(k_trsm_cols_llnu64, dim3((unsigned)omx_cdiv(nn, 64)), dim3(256), 0, st, nn,
-                                   T + i + i * ldt, ldt, B + i + c * ldb, ldb);
-                OMX_LAUNCH_CHECK();
+        for (int64_t I0 = 0; I0 < m; I0 += kOB) {
+            const int64_t w = std::min(kOB, m - I0);
+            for (int64_t i = I0; i < I0 + w; i += kNB) {
+                for (int64_t c = 0; c < n; c += nmax) {
+                    const int64_t nn = (n - c < nmax) ? n - c : nmax;
+                    hipLaunchKernelGGL(k_trsm_cols_llnu64, dim3((unsigned)omx_cdiv(nn, 64)), dim3(256), 0, st, nn,
+                                       T + i + i * ldt, ldt, B + i + c * ldb, ldb);
+                    OMX_LAUNCH_CHECK();
+                }
+                const int64_t inner = I0 + w - i - kNB;
+                if (inner > 0) RC(sgemm_big(inner, n, kNB, -1.0f, T + (i + kNB) + i * ldt, ldt, B + i, ldb, 1.0f,
+                                            B + i + kNB, ldb, st));
             }
-            const int64_t rest = m - i - kNB;
-            if (rest > 0) {
-                int rc = sgemm_big(rest, n, kNB, -1.0f, T + (i + kNB) + i * ldt, ldt, B + i, ldb, 1.0f,
-                                   B + i + kNB, ldb, st);
-                if (rc) return rc;
-            }
+            const int64_t rest = m - I0 - w;
+            if (rest > 0) RC(sgemm_big(rest, n, w, -1.0f, T + (I0 + w) + I0 * ldt, ldt, B + I0, ldb, 1.0f,
+                                       B + I0 + w, ldb, st));
         }
         return OMX_OK;
     }
@@ -512,32 +725,41 @@ int omx_trsm_identity_f32(int variant, int64_t n, const float *T, int64_t ldt, f
     hipStream_t st = omx_stream(stream);
     hipLaunchKernelGGL(k_identity, dim3((unsigned)omx_cdiv(n, 256), (unsigned)n), dim3(256), 0, st, n, X, ldx);
     OMX_LAUNCH_CHECK();
+    // the blocking of omx_trsm_f32, the products restricted to the rows
+    // that can be nonzero, the kernels chosen for all n rows: the results are
+    // those of omx_trsm_f32 on the identity
     if (variant == OMX_TRSM_RUNN) {
         // block column j of I * U^-1 is zero below row j + 63
-        for (int64_t j = 0; j < n; j += kNB) {
-            const int64_t rows = j + kNB;
-            hipLaunchKernelGGL(k_trsm_rows_runn64, dim3((unsigned)omx_cdiv(rows, 64)), dim3(256), 0, st, rows,
-                               T + j + j * ldt, ldt, X + j * ldx, ldx);
-            OMX_LAUNCH_CHECK();
-            const int64_t rest = n - j - kNB;
-            if (rest > 0) {
-                int rc = sgemm_big(rows, rest, kNB, -1.0f, X + j * ldx, ldx, T + j + (j + kNB) * ldt, ldt, 1.0f,
-                                   X + (j + kNB) * ldx, ldx, st);
-                if (rc) return rc;
+        for (int64_t J0 = 0; J0 < n; J0 += kOB) {
+            const int64_t w = std::min(kOB, n - J0);
+            for (int64_t j = J0; j < J0 + w; j += kNB) {
+                const int64_t rows = j + kNB;
+                hipLaunchKernelGGL(k_trsm_rows_runn64, dim3((unsigned)omx_cdiv(rows, 64)), dim3(256), 0, st, rows,
+                                   T + j + j * ldt, ldt, X + j * ldx, ldx);
+                OMX_LAUNCH_CHECK();
+                const int64_t inner = J0 + w - j - kNB;
+                if (inner > 0) RC(sgemm_big(rows, inner, kNB, -1.0f, X + j * ldx, ldx, T + j + (j + kNB) * ldt, ldt,
+                                            1.0f, X + (j + kNB) * ldx, ldx, st, n));
             }
+            const int64_t rest = n - J0 - w;
+            if (rest > 0) RC(sgemm_big(J0 + w, rest, w, -1.0f, X + J0 * ldx, ldx, T + J0 + (J0 + w) * ldt, ldt, 1.0f,
+                                       X + (J0 + w) * ldx, ldx, st, n));
         }
         return OMX_OK;
     }
     // block column j of I * L^-1 is zero above row j
-    for (int64_t j = n - kNB; j >= 0; j -= kNB) {
-        const int64_t rows = n - j;
-        hipLaunchKernelGGL(k_trsm_rows_rlnu64, dim3((unsigned)omx_cdiv(rows, 256)), dim3(256), 0, st, rows,
-                           T + j + j * ldt, ldt, X + j + j * ldx, ldx);
-        OMX_LAUNCH_CHECK();
-        if (j > 0) {
-            int rc = sgemm_big(rows, j, kNB, -1.0f, X + j + j * ldx, ldx, T + j, ldt, 1.0f, X + j, ldx, st);
-            if (rc) return rc;
+    for (int64_t J1 = n; J1 > 0; J1 -= kOB) {
+        const int64_t J0 = std::max<int64_t>(0, J1 - kOB);
+        for (int64_t j = J1 - kNB; j >= J0; j -= kNB) {
+            const int64_t rows = n - j;
+            hipLaunchKernelGGL(k_trsm_rows_rlnu64, dim3((unsigned)omx_cdiv(rows, 256)), dim3(256), 0, st, rows,
+                               T + j + j * ldt, ldt, X + j + j * ldx, ldx);
+            OMX_LAUNCH_CHECK();
+            if (j > J0) RC(sgemm_big(rows, j - J0, kNB, -1.0f, X + j + j * ldx, ldx, T + j + J0 * ldt, ldt, 1.0f,
+                                     X + j + J0 * ldx, ldx, st, n));
         }
+        if (J0 > 0) RC(sgemm_big(n - J0, J0, J1 - J0, -1.0f, X + J0 + J0 * ldx, ldx, T + J0, ldt, 1.0f, X + J0, ldx,
+                                 st, n));
     }
     return OMX_OK;
 }

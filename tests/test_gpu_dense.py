@@ -45,6 +45,31 @@ def test_sgemm_matches_numpy():
     assert np.all(np.abs(got - exact) <= bound)
 
 
+@pytest.mark.parametrize("m,n,k,beta", [(2100, 2050, 80, 1.0), (2176, 1930, 208, 0.0), (4100, 1030, 16, 0.5)])
+def test_sgemm_large_matches_numpy(m, n, k, beta):
+    """Products of at least 2^22 entries with 16-byte aligned A and B run on
+    the matrix cores: ragged edges in both dimensions, k in stages of 64 with
+    a shorter last stage (or a single stage of 16), and beta = 0 with C full
+    of NaN (C must not be read)."""
+    torch_cuda()
+    print(f"sgemm large m={m} n={n} k={k} beta={beta}", flush=True)
+    L = _L()
+    rng = np.random.default_rng(m + k)
+    A = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    B = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+    C = rng.uniform(-1, 1, (m, n)).astype(np.float32) if beta else np.full((m, n), np.nan, np.float32)
+    Ad, lda = _fdev(A, m + 4)
+    Bd, ldb = _fdev(B, k + 4)
+    Cd, ldc = _fdev(C, m + 1)
+    assert L.omx_sgemm_f32(m, n, k, -1.0, ptr(Ad), lda, ptr(Bd), ldb, beta, ptr(Cd), ldc, None) == 0
+    got = colmajor_to_numpy(Cd, m, n, ldc)
+    ab = A.astype(np.float64) @ B.astype(np.float64)
+    c0 = C.astype(np.float64) if beta else np.zeros((m, n))
+    exact = beta * c0 - ab
+    bound = (k + 2) * 2.0**-24 * (np.abs(beta * c0) + np.abs(A).astype(np.float64) @ np.abs(B))
+    assert np.all(np.abs(got - exact) <= bound)
+
+
 @pytest.mark.parametrize("n", [256, 512, 2560])
 def test_getrf_matches_oracle(orc, n):
     torch = torch_cuda()
