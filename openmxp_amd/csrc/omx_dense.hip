@@ -96,9 +96,9 @@ k_sgemm_nn(int64_t M, int64_t N, int64_t K, float alpha, const float *__restrict
 
 // ---------------------------------------------------------------------------
 // The same product on the matrix cores (v_mfma_f32_32x32x2_f32), for the
-// large updates: a 128 x 128 tile per workgroup of four waves (a 64 x 64
-// quadrant each, 2 x 2 MFMA blocks of 32 x 32), k staged through LDS up to
-// 64 at a time, the next stage's global loads in flight during the current
+// large updates: a kT x kT tile (128, or 64 where 128-tiles would leave CUs
+// idle) per workgroup of four waves (a kT/2 square quadrant each, made of
+// 32 x 32 MFMA blocks), k staged through LDS up to 64 at a time, the next stage's global loads in flight during the current
 // stage's MFMAs.  The MFMA's first operand takes B (lane l: column l % 32),
 // the second A (lane l: row l % 32), so every accumulator register of lane l
 // belongs to row l % 32 of its block and a half-wave's C access is 32
@@ -109,28 +109,30 @@ k_sgemm_nn(int64_t M, int64_t N, int64_t K, float alpha, const float *__restrict
 // ---------------------------------------------------------------------------
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-constexpr int kMT = 128;        // tile rows and columns
 constexpr int kMK = 64;         // k per LDS stage
 constexpr int kBLd = kMK + 4;   // row stride of Bs: conflict-free 16-byte reads
 
-template <bool kOneStage>
-__global__ void __launch_bounds__(256)
+template <int kT, bool kOneStage>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_sgemm_mfma(int64_t M, int64_t N, int64_t K, float alpha, const float *__restrict__ A, int64_t lda,
              const float *__restrict__ B, int64_t ldb, float beta, float *__restrict__ C, int64_t ldc,
              int64_t sA, int64_t sB, int64_t sC) {
-    __shared__ float As[kMK][kMT];   // As[k][i] = A(i0 + i, k0 + k)
-    __shared__ float Bs[kMT][kBLd];  // Bs[j][k] = B(k0 + k, j0 + j)
+    constexpr int kB = kT / 64;       // MFMA blocks per quadrant side
+    constexpr int kS = kT / 16;       // float4 loads per thread per operand and stage
+    constexpr int kAK = 1024 / kT;    // stage columns between a thread's A loads
+    __shared__ float As[kMK][kT];     // As[k][i] = A(i0 + i, k0 + k)
+    __shared__ float Bs[kT][kBLd];    // Bs[j][k] = B(k0 + k, j0 + j)
     A += (int64_t)blockIdx.z * sA;
     B += (int64_t)blockIdx.z * sB;
     C += (int64_t)blockIdx.z * sC;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l32 = lane & 31, h = lane >> 5;
-    const int64_t i0 = (int64_t)blockIdx.x * kMT, j0 = (int64_t)blockIdx.y * kMT;
-    const int wi = (wave & 1) * 64, wj = (wave >> 1) * 64;
-    // global -> LDS: A rows ar .. ar+3 at stage columns ak + 8 s; B column
+    const int64_t i0 = (int64_t)blockIdx.x * kT, j0 = (int64_t)blockIdx.y * kT;
+    const int wi = (wave & 1) * (kT / 2), wj = (wave >> 1) * (kT / 2);
+    // global -> LDS: A rows ar .. ar+3 at stage columns ak + kAK s; B column
     // bj + 16 s, stage rows bk .. bk+3.  Rows (columns) past M (N) are
     // clamped: their products land only in results that are not stored.
-    const int ar = 4 * (tid & 31), ak = tid >> 5;
+    const int ar = 4 * (tid % (kT / 4)), ak = tid / (kT / 4);
     const int bk = 4 * (tid & 15), bj = tid >> 4;
     const int64_t arow = i0 + ar;
     const bool afull = arow + 3 < M;
@@ -139,34 +141,34 @@ k_sgemm_mfma(int64_t M, int64_t N, int64_t K, float alpha, const float *__restri
     if (ar1 >= M) ar1 = M - 1;
     if (ar2 >= M) ar2 = M - 1;
     if (ar3 >= M) ar3 = M - 1;
-    const float *Bcol[8];
+    const float *Bcol[kS];
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
+    for (int s = 0; s < kS; ++s) {
         int64_t j = j0 + bj + 16 * s;
         if (j >= N) j = N - 1;
         Bcol[s] = B + j * ldb;
     }
-    f32x4 ra[8], rb[8];
+    f32x4 ra[kS], rb[kS];
     // stage k0 .. k0 + kc into ra / rb.  Every load is unconditional: stage
     // columns (rows) at or past kc read column (row) kc - 1 (kc - 4) instead,
     // and are not stored to LDS.
 #define OMX_SGEMM_GLOAD(k0, kc)                                                                            \
     {                                                                                                      \
-        _Pragma("unroll") for (int s = 0; s < 8; ++s) {                                                    \
-            const int kk = ak + 8 * s < (kc) ? ak + 8 * s : (kc) - 1;                                      \
+        _Pragma("unroll") for (int s = 0; s < kS; ++s) {                                                   \
+            const int kk = ak + kAK * s < (kc) ? ak + kAK * s : (kc) - 1;                                  \
             const float *p = A + ((k0) + kk) * lda;                                                        \
             ra[s] = afull ? *reinterpret_cast<const f32x4 *>(p + arow)                                    \
                           : f32x4{p[arow0], p[ar1], p[ar2], p[ar3]};                                 \
         }                                                                                                  \
         const int kb4 = bk < (kc) ? bk : (kc) - 4;                                                         \
-        _Pragma("unroll") for (int s = 0; s < 8; ++s)                                                      \
+        _Pragma("unroll") for (int s = 0; s < kS; ++s)                                                     \
             rb[s] = *reinterpret_cast<const f32x4 *>(Bcol[s] + (k0) + kb4);                               \
     }
-    f32x16 acc[2][2];
+    f32x16 acc[kB][kB];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < kB; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < kB; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
     int64_t k0 = 0;
@@ -178,14 +180,14 @@ k_sgemm_mfma(int64_t M, int64_t N, int64_t K, float alpha, const float *__restri
     // cv[a][b][v] sits where acc[a][b][v] does (see below).  With several
     // stages the staging registers are live through the loop and C is read
     // in the epilogue.
-    float cv[2][2][16];
+    float cv[kB][kB][16];
     if (kOneStage && beta != 0.0f) {
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
+        for (int a = 0; a < kB; ++a) {
             int64_t i = i0 + wi + 32 * a + l32;
             if (i >= M) i = M - 1;
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+            for (int b = 0; b < kB; ++b)
 #pragma unroll
                 for (int v = 0; v < 16; ++v) {
                     int64_t j = j0 + wj + 32 * b + 4 * h + 8 * (v >> 2) + (v & 3);
@@ -196,11 +198,11 @@ k_sgemm_mfma(int64_t M, int64_t N, int64_t K, float alpha, const float *__restri
     }
     for (;;) {
 #pragma unroll
-        for (int s = 0; s < 8; ++s)
-            if (ak + 8 * s < kc) *reinterpret_cast<f32x4 *>(&As[ak + 8 * s][ar]) = ra[s];
+        for (int s = 0; s < kS; ++s)
+            if (ak + kAK * s < kc) *reinterpret_cast<f32x4 *>(&As[ak + kAK * s][ar]) = ra[s];
         if (bk < kc) {
 #pragma unroll
-            for (int s = 0; s < 8; ++s) *reinterpret_cast<f32x4 *>(&Bs[bj + 16 * s][bk]) = rb[s];
+            for (int s = 0; s < kS; ++s) *reinterpret_cast<f32x4 *>(&Bs[bj + 16 * s][bk]) = rb[s];
         }
         __syncthreads();
         const int64_t kn = k0 + kc;
@@ -208,16 +210,19 @@ k_sgemm_mfma(int64_t M, int64_t N, int64_t K, float alpha, const float *__restri
         if (kcn) OMX_SGEMM_GLOAD(kn, kcn);
         const int half = kc >> 1, kb = h * half;
         for (int s = 0; s < half; s += 4) {
-            const f32x4 y0 = *reinterpret_cast<const f32x4 *>(&Bs[wj + l32][kb + s]);
-            const f32x4 y1 = *reinterpret_cast<const f32x4 *>(&Bs[wj + 32 + l32][kb + s]);
+            f32x4 y[kB];
+#pragma unroll
+            for (int b = 0; b < kB; ++b) y[b] = *reinterpret_cast<const f32x4 *>(&Bs[wj + 32 * b + l32][kb + s]);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const float x0 = As[kb + s + u][wi + l32];
-                const float x1 = As[kb + s + u][wi + 32 + l32];
-                acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(y0[u], x0, acc[0][0], 0, 0, 0);
-                acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(y1[u], x0, acc[0][1], 0, 0, 0);
-                acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(y0[u], x1, acc[1][0], 0, 0, 0);
-                acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(y1[u], x1, acc[1][1], 0, 0, 0);
+                float x[kB];
+#pragma unroll
+                for (int a = 0; a < kB; ++a) x[a] = As[kb + s + u][wi + 32 * a + l32];
+#pragma unroll
+                for (int a = 0; a < kB; ++a)
+#pragma unroll
+                    for (int b = 0; b < kB; ++b)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(y[b][u], x[a], acc[a][b], 0, 0, 0);
             }
         }
         __syncthreads();
@@ -228,11 +233,11 @@ k_sgemm_mfma(int64_t M, int64_t N, int64_t K, float alpha, const float *__restri
 #undef OMX_SGEMM_GLOAD
     // acc[a][b][v]: row i0 + wi + 32 a + l32, column j0 + wj + 32 b + 8 (v / 4) + 4 h + v % 4
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
+    for (int a = 0; a < kB; ++a) {
         const int64_t i = i0 + wi + 32 * a + l32;
         if (i >= M) continue;
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < kB; ++b) {
             const int64_t jb = j0 + wj + 32 * b + 4 * h;
             if (!kOneStage && beta != 0.0f) {
 #pragma unroll
@@ -495,6 +500,9 @@ __global__ void k_set_int(int *p, int v) { *p = v; }
 // The MFMA kernel takes products with at least this many C entries per
 // launch (2^18 = 512 x 512; smaller ones are latency-bound and the 64 x 64
 // VALU tiles spread them over more CUs) when its alignment conditions hold.
+#ifndef OMX_SGEMM_MFMA_T64
+#define OMX_SGEMM_MFMA_T64 1024
+#endif
 #ifndef OMX_SGEMM_MFMA_MIN
 #define OMX_SGEMM_MFMA_MIN (int64_t{1} << 18)
 #endif
@@ -508,13 +516,24 @@ bool use_mfma(int64_t m, int64_t n, int64_t k, int64_t batch, const float *A, in
 
 int sgemm_mfma(int64_t m, int64_t n, int64_t k, float alpha, const float *A, int64_t lda, int64_t sA, const float *B,
                int64_t ldb, int64_t sB, float beta, float *C, int64_t ldc, int64_t sC, int64_t batch, hipStream_t st) {
-    const dim3 grid((unsigned)omx_cdiv(m, kMT), (unsigned)omx_cdiv(n, kMT), (unsigned)batch);
-    if (k <= kMK)
-        hipLaunchKernelGGL(k_sgemm_mfma<true>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA,
-                           sB, sC);
-    else
-        hipLaunchKernelGGL(k_sgemm_mfma<false>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA,
-                           sB, sC);
+    // 64-tiles where 128-tiles would give fewer than 1024 workgroups (four
+    // per CU): the inverse merges (few tiles, deep k) and the mid-size LU
+    // updates finish sooner on four times as many, smaller workgroups
+    const bool small = omx_cdiv(m, 128) * omx_cdiv(n, 128) * batch < OMX_SGEMM_MFMA_T64;
+    const int t = small ? 64 : 128;
+    const dim3 grid((unsigned)omx_cdiv(m, t), (unsigned)omx_cdiv(n, t), (unsigned)batch);
+    const bool one = k <= kMK;
+#define OMX_SGEMM_LAUNCH(T, ONE)                                                                              \
+    hipLaunchKernelGGL((k_sgemm_mfma<T, ONE>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, \
+                       ldc, sA, sB, sC)
+    if (small) {
+        if (one) OMX_SGEMM_LAUNCH(64, true);
+        else OMX_SGEMM_LAUNCH(64, false);
+    } else {
+        if (one) OMX_SGEMM_LAUNCH(128, true);
+        else OMX_SGEMM_LAUNCH(128, false);
+    }
+#undef OMX_SGEMM_LAUNCH
     OMX_LAUNCH_CHECK();
     return OMX_OK;
 }

@@ -45,12 +45,14 @@ def test_sgemm_matches_numpy():
     assert np.all(np.abs(got - exact) <= bound)
 
 
-@pytest.mark.parametrize("m,n,k,beta", [(2100, 2050, 80, 1.0), (2176, 1930, 208, 0.0), (4100, 1030, 16, 0.5)])
+@pytest.mark.parametrize("m,n,k,beta", [(2100, 2050, 80, 1.0), (2176, 1930, 208, 0.0), (4100, 1030, 16, 0.5),
+                                        (600, 520, 64, 1.0), (4200, 4100, 64, 1.0), (4300, 4000, 96, 0.0)])
 def test_sgemm_large_matches_numpy(m, n, k, beta):
-    """Products of at least 2^22 entries with 16-byte aligned A and B run on
-    the matrix cores: ragged edges in both dimensions, k in stages of 64 with
-    a shorter last stage (or a single stage of 16), and beta = 0 with C full
-    of NaN (C must not be read)."""
+    """Products of at least 512 x 512 entries with 16-byte aligned A and B
+    run on the matrix cores, in 64-tiles below 1024 tiles of 128 and in
+    128-tiles from there (the last two cases): ragged edges in both
+    dimensions, k in stages of 64 with a shorter last stage (or a single
+    stage), and beta = 0 with C full of NaN (C must not be read)."""
     torch_cuda()
     print(f"sgemm large m={m} n={n} k={k} beta={beta}", flush=True)
     L = _L()
