@@ -267,28 +267,23 @@ __device__ __forceinline__ float lane_bcast(float v, int l) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
 }
 
-__global__ void __launch_bounds__(kNB)
-k_getrf_diag64(float *__restrict__ A, int64_t lda, int64_t row0, int *__restrict__ info) {
-    const int i = threadIdx.x;
-    float r[kNB];
-#pragma unroll
-    for (int j = 0; j < kNB; ++j) r[j] = A[i + (int64_t)j * lda];
+// Lane i of the calling wave: row i of the 64 x 64 block in r, factored in
+// place (multipliers below the diagonal, U on and above it).  info (may be
+// null): the smallest row0 + k + 1 with a zero pivot u_kk.
+__device__ __forceinline__ void lu64_wave(float (&r)[kNB], int i, int64_t row0, int *info) {
 #pragma unroll
     for (int k = 0; k < kNB; ++k) {
         const float ukk = lane_bcast(r[k], k);
         if (i == 0 && ukk == 0.0f && info) atomicMin(info, (int)(row0 + k + 1));
         const float inv = 1.0f / ukk;
-        const float l = r[k] * inv;
         const bool below = i > k;
+        // rows at or above k take multiplier 0: fma(-0, u, r) leaves r as
+        // it is (up to the sign of a zero), with no select per entry
+        const float l = below ? r[k] * inv : 0.0f;
         if (below) r[k] = l;
 #pragma unroll
-        for (int j = k + 1; j < kNB; ++j) {
-            const float u = lane_bcast(r[j], k);
-            r[j] = below ? __builtin_fmaf(-l, u, r[j]) : r[j];
-        }
+        for (int j = k + 1; j < kNB; ++j) r[j] = __builtin_fmaf(-l, lane_bcast(r[j], k), r[j]);
     }
-#pragma unroll
-    for (int j = 0; j < kNB; ++j) A[i + (int64_t)j * lda] = r[j];
 }
 
 // The 64 x 64 triangle T (column-major, leading dimension ldt) into LDS:
@@ -401,24 +396,51 @@ k_trsm_cols_llnu64(int64_t n, const float *__restrict__ L, int64_t ldl, float *_
     if (j < n) col_llnu64(j, threadIdx.x & 3, l, Y, ldy);
 }
 
-// Both solves of one 64-column step of the blocked LU in one launch, with
-// the factored diagonal block D at Ajj: workgroups [0, nrw) take the rows
-// below D (RUNN with D's U), the rest the columns right of D (LLNU with D's
-// unit L).  rest = rows below = columns right of D; 64 per workgroup.
+// One 64-column step of the blocked LU in one launch.  Every workgroup
+// factors the 64 x 64 diagonal block D at Ajj itself (wave 0, lu64_wave;
+// all get the same factors) into LDS; workgroup 0 also writes D back and
+// reports zero pivots.  Then workgroups [0, nrw) solve the rows below D
+// (RUNN with D's U), the others the columns right of D (LLNU with D's unit
+// L), 64 per workgroup, a quad per row or column; rest = rows below =
+// columns right of D.  Each quad's entries are loaded before the
+// factorisation, so the loads overlap it.
 __global__ void __launch_bounds__(256)
-k_getrf_trsm64(int64_t rest, int nrw, float *__restrict__ Ajj, int64_t lda) {
+k_getrf_step64(int64_t rest, int nrw, float *__restrict__ Ajj, int64_t lda, int64_t row0, int *__restrict__ info) {
     __shared__ float d[kNB][kNB + 1];
     __shared__ float rd[kNB];
-    load_tri64(Ajj, lda, d);
-    diag_recip64(d, rd);
-    const int q = threadIdx.x & 3;
-    if ((int)blockIdx.x < nrw) {
-        const int64_t i = (int64_t)blockIdx.x * 64 + (threadIdx.x >> 2);
-        if (i < rest) row_runn64(i, q, d, rd, Ajj + kNB, lda);
-    } else {
-        const int64_t j = (int64_t)(blockIdx.x - nrw) * 64 + (threadIdx.x >> 2);
-        if (j < rest) col_llnu64(j, q, d, Ajj + kNB * lda, lda);
+    const int tid = threadIdx.x, q = tid & 3;
+    const bool rows = (int)blockIdx.x < nrw;
+    const int64_t e = (int64_t)(rows ? (int)blockIdx.x : (int)blockIdx.x - nrw) * 64 + (tid >> 2);
+    const bool act = e < rest;
+    // row e below D: entries (e, q + 4 t); column e right of D: (q + 4 t, e)
+    float *base = rows ? Ajj + kNB + e : Ajj + (kNB + e) * lda;
+    const int64_t step = rows ? lda : 1;
+    float x[16];
+    if (act) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) x[t] = base[(q + 4 * t) * step];
     }
+    if (tid < kNB) {
+        float r[kNB];
+#pragma unroll
+        for (int j = 0; j < kNB; ++j) r[j] = Ajj[tid + (int64_t)j * lda];
+        lu64_wave(r, tid, row0, blockIdx.x == 0 ? info : nullptr);
+#pragma unroll
+        for (int j = 0; j < kNB; ++j) d[tid][j] = r[j];
+        if (blockIdx.x == 0) {
+#pragma unroll
+            for (int j = 0; j < kNB; ++j) Ajj[tid + (int64_t)j * lda] = r[j];
+        }
+    }
+    __syncthreads();
+    diag_recip64(d, rd);
+    if (!act) return;
+    if (rows)
+        runn_all(x, q, d, rd, std::make_integer_sequence<int, kNB>{});
+    else
+        llnu_all(x, q, d, std::make_integer_sequence<int, kNB>{});
+#pragma unroll
+    for (int t = 0; t < 16; ++t) base[(q + 4 * t) * step] = x[t];
 }
 
 // Rows: X(i, :) <- X(i, :) * L^-1 for a 64 x 64 unit lower L (RLNU),
@@ -603,13 +625,12 @@ int omx_getrf_nopiv_f32(int64_t n, float *A, int64_t lda, int *info, void *strea
     }
     for (int64_t j = 0; j < n; j += kNB) {
         float *Ajj = A + j + j * lda;
-        hipLaunchKernelGGL(k_getrf_diag64, dim3(1), dim3(kNB), 0, st, Ajj, lda, j, info);
-        OMX_LAUNCH_CHECK();
         const int64_t rest = n - j - kNB;
-        if (rest <= 0) break;
         const int nrw = (int)omx_cdiv(rest, 64);
-        hipLaunchKernelGGL(k_getrf_trsm64, dim3((unsigned)(2 * nrw)), dim3(256), 0, st, rest, nrw, Ajj, lda);
+        hipLaunchKernelGGL(k_getrf_step64, dim3((unsigned)std::max(1, 2 * nrw)), dim3(256), 0, st, rest, nrw, Ajj, lda, j,
+                           info);
         OMX_LAUNCH_CHECK();
+        if (rest <= 0) break;
         int rc = sgemm(rest, rest, kNB, -1.0f, Ajj + kNB, lda, Ajj + kNB * lda, lda, 1.0f,
                        Ajj + kNB + kNB * lda, lda, st);
         if (rc) return rc;
