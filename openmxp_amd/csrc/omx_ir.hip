@@ -119,21 +119,36 @@ __global__ void k_zero(int64_t n, double *__restrict__ y) {
 // Triangular solves with the FP32 factors, FP64 vector and arithmetic.
 //
 // The matrix is walked in blocks of kTB = 256 rows/columns.  One launch per
-// block column J (solved values x_J final in x):
+// block column J (solved values x_J final in x), workgroups of 1024
+// threads (16 waves):
 //   workgroup 0   rows of the next diagonal block D: r_D -= M(D, J) x_J, then
-//                 the diagonal block is solved in LDS (4 sub-blocks of 64,
-//                 each a lane-to-lane chain in one wave, followed by the
-//                 update of the later sub-blocks' rows), x_D written back;
+//                 D is solved in LDS: four sub-blocks of 64, each a 64-step
+//                 lane-to-lane chain in one wave, while twelve more waves
+//                 fold each solved sub-block into the later sub-blocks' rows;
 //   workgroup w   the remaining rows beyond D, 256 per workgroup:
 //                 r -= M(rows, J) x_J.
-// So the diagonal chain of block D runs concurrently with the bulk update by
-// block J, and the solve reads each FP32 factor entry exactly once (HBM-bound:
-// 2 n^2 bytes per triangle).  Summation order per row and block column: the
-// four waves each sum 64 columns in order, the partial sums are added in
-// wave order, the total is subtracted from r.
+// A wave takes 16 of the 256 columns: its sixteen 16-byte loads per lane
+// are in flight together, so a workgroup's 256 KB of the factor arrive in
+// one round trip and the launch is bound by HBM rather than by the latency
+// of one workgroup's reads.  Summation order per row and block column: each
+// wave sums its 16 columns in order, the 16 partial sums are added in wave
+// order, the total is subtracted from r.  Every FP32 factor entry is read
+// once (2 n^2 bytes per triangle).
 // ---------------------------------------------------------------------------
 constexpr int kTB = 256;
 constexpr int kSB = 64;
+constexpr int kTW = 16;         // waves per workgroup
+constexpr int kTT = 64 * kTW;   // threads per workgroup
+constexpr int kTC = kTB / kTW;  // columns per wave
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct TrsvShared {
+    double xv[kTB];          // x_J
+    double xs[kTB];          // r_D, then x_D
+    double part[kTW][kTB];   // per-wave partial sums of rows_dot
+    double pp[4][4][kSB];    // diagonal block: [sub-block][column chunk][row] sums
+};
 
 // Lane l's value of v in every lane (l uniform): two readlanes.
 __device__ __forceinline__ double bcast_lane(double v, int l) {
@@ -143,163 +158,163 @@ __device__ __forceinline__ double bcast_lane(double v, int l) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// sum over the block's columns of M(row, c) x_c for 256 rows from r0 (rows
-// >= rmax ignored) and ncols <= 256 columns from c0; xv (LDS) holds x of the
-// columns.  Wave w takes columns [64 w, 64 w + 64), lane l rows 4l .. 4l+3
+// sum over the block's ncols <= 256 columns from c0 of M(row, c) x_c for
+// the 256 rows from r0 (rows >= rmax ignored); sh.xv holds x of the
+// columns.  Wave w takes columns [16 w, 16 w + 16), lane l rows 4l .. 4l+3
 // (one 16-byte load per column when kVec).  Returns the row's total in
-// thread t = row - r0 (after the LDS reduction).
+// thread t = row - r0 for t < 256 (0 elsewhere).
 template <bool kVec>
-__device__ __forceinline__ double block_rows_dot(const float *__restrict__ A, int64_t lda, int64_t r0,
-                                                 int64_t rmax, int64_t c0, int ncols, const double *xv,
-                                                 double (*part)[kTB]) {
+__device__ __forceinline__ double rows_dot(const float *__restrict__ A, int64_t lda, int64_t r0, int64_t rmax,
+                                           int64_t c0, int ncols, TrsvShared &sh) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int cb = wave * kSB;
-    const int ce = min(ncols, cb + kSB);
+    const int cb = wave * kTC;
+    const int ce = min(ncols, cb + kTC);
     const int64_t row = r0 + 4 * lane;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     if (kVec && row + 3 < rmax) {
         const float *p = A + row + (c0 + cb) * lda;
-        int c = cb;
-        for (; c + 8 <= ce; c += 8) {
-            float4 v[8];
+        if (ce - cb == kTC) {
+            f32x4 v[kTC];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const float4 *>(p + (int64_t)(c - cb + u) * lda);
+            for (int u = 0; u < kTC; ++u) v[u] = *reinterpret_cast<const f32x4 *>(p + (int64_t)u * lda);
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const double xc = xv[c + u];
+            for (int u = 0; u < kTC; ++u) {
+                const double xc = sh.xv[cb + u];
                 s0 = __builtin_fma((double)v[u].x, xc, s0);
                 s1 = __builtin_fma((double)v[u].y, xc, s1);
                 s2 = __builtin_fma((double)v[u].z, xc, s2);
                 s3 = __builtin_fma((double)v[u].w, xc, s3);
             }
-        }
-        for (; c < ce; ++c) {
-            const float4 v = *reinterpret_cast<const float4 *>(p + (int64_t)(c - cb) * lda);
-            const double xc = xv[c];
-            s0 = __builtin_fma((double)v.x, xc, s0);
-            s1 = __builtin_fma((double)v.y, xc, s1);
-            s2 = __builtin_fma((double)v.z, xc, s2);
-            s3 = __builtin_fma((double)v.w, xc, s3);
+        } else {
+            for (int c = cb; c < ce; ++c) {
+                const f32x4 v = *reinterpret_cast<const f32x4 *>(p + (int64_t)(c - cb) * lda);
+                const double xc = sh.xv[c];
+                s0 = __builtin_fma((double)v.x, xc, s0);
+                s1 = __builtin_fma((double)v.y, xc, s1);
+                s2 = __builtin_fma((double)v.z, xc, s2);
+                s3 = __builtin_fma((double)v.w, xc, s3);
+            }
         }
     } else {
         for (int c = cb; c < ce; ++c) {
             const float *q = A + row + (c0 + c) * lda;
-            const double xc = xv[c];
+            const double xc = sh.xv[c];
             if (row < rmax) s0 = __builtin_fma((double)q[0], xc, s0);
             if (row + 1 < rmax) s1 = __builtin_fma((double)q[1], xc, s1);
             if (row + 2 < rmax) s2 = __builtin_fma((double)q[2], xc, s2);
             if (row + 3 < rmax) s3 = __builtin_fma((double)q[3], xc, s3);
         }
     }
-    part[wave][4 * lane] = s0;
-    part[wave][4 * lane + 1] = s1;
-    part[wave][4 * lane + 2] = s2;
-    part[wave][4 * lane + 3] = s3;
+    sh.part[wave][4 * lane] = s0;
+    sh.part[wave][4 * lane + 1] = s1;
+    sh.part[wave][4 * lane + 2] = s2;
+    sh.part[wave][4 * lane + 3] = s3;
     __syncthreads();
-    const double t = ((part[0][tid] + part[1][tid]) + part[2][tid]) + part[3][tid];
+    double t = 0.0;
+    if (tid < kTB) {
+        t = sh.part[0][tid];
+#pragma unroll
+        for (int w = 1; w < kTW; ++w) t += sh.part[w][tid];
+    }
     __syncthreads();
     return t;
 }
 
-// Lane's row `row` of the diagonal block at d0 (order dim), the 64 columns
-// from c0: row and column indices are clamped into the block, so every load
-// is unconditional and in bounds (all 64 can be in flight at once); entries
-// outside the block are masked where they are used.
-__device__ __forceinline__ void load_row64(const float *__restrict__ A, int64_t lda, int64_t d0, int dim,
-                                           int row, int c0, float (&a)[kSB]) {
-    const float *p = A + d0 + min(row, dim - 1) + d0 * lda;
-#pragma unroll
-    for (int c = 0; c < kSB; ++c) a[c] = p[(int64_t)min(c0 + c, dim - 1) * lda];
-}
-
-// x_D <- L_DD^-1 r_D for the unit-lower diagonal block at d0 of order dim
-// (<= 256); r_D in xs (LDS).  Wave w owns rows [64 w, 64 w + 64) (lane l:
-// row 64 w + l) and holds its diagonal sub-block's rows in registers from
-// the start.  Sub-block sb: wave sb runs the 64-step chain (x_c read from
-// lane c), then every later wave subtracts its rows' product with x_sb,
-// using the off-diagonal sub-block it prefetched during the previous stage.
-__device__ __forceinline__ void diag_solve_lower(const float *__restrict__ A, int64_t lda, int64_t d0, int dim,
-                                                 double *xs) {
+// x_D <- T_DD^-1 r_D for the diagonal block at d0 of order dim (<= 256),
+// unit lower (kUpper false, sub-blocks first to last) or upper non-unit
+// (kUpper, last to first); r_D in sh.xs.  Roles:
+//   waves 0-3    wave s runs sub-block s's 64-step chain at stage s (lane l:
+//                row 64 s + l, its 64 entries of the diagonal sub-block in
+//                registers, x_c read from lane c);
+//   waves 4-15   wave 4 + 4 j + c accumulates, for the rows of target
+//                sub-block t (lower t = j + 1, upper t = j), the products
+//                with columns [16 c, 16 c + 16) of every sub-block solved
+//                before t, each right after that sub-block's stage, and
+//                hands its sum to t's chain, which subtracts the four chunk
+//                sums (in chunk order) before it starts.
+// All factor entries are loaded at the start, with row and column indices
+// clamped into the block so every load is unconditional; entries outside
+// the block meet x = 0 or are masked.  The upper chain divides by u_ii as
+// q0 = r y, q = q0 + (r - u q0) y with y = RN(1/u): the correctly rounded
+// quotient (Markstein's correction with a correctly rounded reciprocal).
+template <bool kUpper>
+__device__ __forceinline__ void diag_solve(const float *__restrict__ A, int64_t lda, int64_t d0, int dim,
+                                           TrsvShared &sh) {
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nsb = (dim + kSB - 1) / kSB;
-    const int i = wave * kSB + lane;
-    float a[kSB], b[kSB];
-    if (wave < nsb) {
-        load_row64(A, lda, d0, dim, i, wave * kSB, a);
-        if (wave > 0) load_row64(A, lda, d0, dim, i, 0, b);
-    }
-#pragma unroll
-    for (int sb = 0; sb < kTB / kSB; ++sb) {
-        if (sb >= nsb) break;
-        const int c0 = sb * kSB;
-        if (wave == sb) {
-            double v = xs[i];
-#pragma unroll
-            for (int c = 0; c < kSB; ++c) {
-                const double m = (lane > c && i < dim) ? (double)a[c] : 0.0;
-                v = __builtin_fma(-bcast_lane(v, c), m, v);
-            }
-            if (i < dim) xs[i] = v;
-        }
-        __syncthreads();
-        if (wave > sb && wave < nsb) {
-            double s = 0.0;
-#pragma unroll
-            for (int c = 0; c < kSB; ++c) s = __builtin_fma(c0 + c < dim ? (double)b[c] : 0.0, xs[c0 + c], s);
-            if (i < dim) xs[i] -= s;
-            if (sb + 1 < wave) load_row64(A, lda, d0, dim, i, c0 + kSB, b);  // next stage's sub-block
-        }
-        __syncthreads();
-    }
-}
-
-// x_D <- U_DD^-1 r_D for the upper non-unit diagonal block (sub-blocks from
-// the last to the first; registers and prefetch as in the lower solve).  The
-// quotient r / u is formed as q0 = r * y, q = q0 + (r - u q0) y with
-// y = 1/u: the correctly rounded quotient (Markstein's correction with a
-// correctly rounded reciprocal).
-__device__ __forceinline__ void diag_solve_upper(const float *__restrict__ A, int64_t lda, int64_t d0, int dim,
-                                                 double *xs) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int nsb = (dim + kSB - 1) / kSB;
-    const int i = wave * kSB + lane;
-    float a[kSB], b[kSB];
+    const float *D = A + d0 + d0 * lda;
+    const bool chain = wave < 4;
+    const int t = chain ? wave : (kUpper ? (wave - 4) / 4 : 1 + (wave - 4) / 4);
+    const int c = (wave - 4) & 3;
+    const int i = kSB * t + lane;
+    const float *Drow = D + min(i, dim - 1);
+    float w[kSB];
     float ud = 1.0f;
-    if (wave < nsb) {
-        load_row64(A, lda, d0, dim, i, wave * kSB, a);
-        ud = A[d0 + min(i, dim - 1) + (d0 + min(i, dim - 1)) * lda];
-        if (wave < nsb - 1) load_row64(A, lda, d0, dim, i, (nsb - 1) * kSB, b);
-    }
+    if (t < nsb) {
+        if (chain) {
 #pragma unroll
-    for (int t = 0; t < kTB / kSB; ++t) {
-        const int sb = nsb - 1 - t;
-        if (sb < 0) break;
-        const int c0 = sb * kSB;
-        if (wave == sb) {
-            const double u = (i < dim) ? (double)ud : 1.0;
-            const double y = 1.0 / u;
-            auto quot = [&](double r) {
-                const double q0 = r * y;
-                return __builtin_fma(__builtin_fma(-u, q0, r), y, q0);
-            };
-            // Step c: every lane forms its quotient, lane c's is x_c; rows
-            // above subtract x_c u(i, c).  Lane c's own v is untouched from
-            // then on, so its final quotient below equals x_c.
-            double v = (i < dim) ? xs[i] : 0.0;
+            for (int q = 0; q < kSB; ++q) w[q] = Drow[(int64_t)min(kSB * t + q, dim - 1) * lda];
+            if (kUpper) ud = Drow[(int64_t)min(i, dim - 1) * lda];
+        } else {
+            // source sub-block s at slot s (lower, s < t) or s - 1 (upper, s > t)
 #pragma unroll
-            for (int c = kSB - 1; c >= 0; --c) {
-                const double m = (lane < c && c0 + c < dim) ? (double)a[c] : 0.0;
-                v = __builtin_fma(-bcast_lane(quot(v), c), m, v);
+            for (int k = 0; k < 3; ++k) {
+                const int s = kUpper ? k + 1 : k;
+                if (kUpper ? (s > t && s < nsb) : s < t) {
+#pragma unroll
+                    for (int u = 0; u < kTC; ++u)
+                        w[kTC * k + u] = Drow[(int64_t)min(kSB * s + kTC * c + u, dim - 1) * lda];
+                }
             }
-            if (i < dim) xs[i] = quot(v);
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int st = kUpper ? 3 - k : k;
+        if (st >= nsb) continue;
+        if (chain && st == t) {
+            const int c0 = kSB * st;
+            const bool fed = kUpper ? st < nsb - 1 : st > 0;
+            double v = (i < dim) ? sh.xs[i] : 0.0;
+            if (fed) v -= ((sh.pp[st][0][lane] + sh.pp[st][1][lane]) + sh.pp[st][2][lane]) + sh.pp[st][3][lane];
+            if (kUpper) {
+                const double u = (i < dim) ? (double)ud : 1.0;
+                const double y = 1.0 / u;
+                auto quot = [&](double r) {
+                    const double q0 = r * y;
+                    return __builtin_fma(__builtin_fma(-u, q0, r), y, q0);
+                };
+                // step q: every lane forms its quotient, lane q's is x_q;
+                // rows above subtract x_q u(i, q).  Lane q's own v is
+                // untouched from then on, so its final quotient equals x_q.
+#pragma unroll
+                for (int q = kSB - 1; q >= 0; --q) {
+                    const double m = (lane < q && c0 + q < dim) ? (double)w[q] : 0.0;
+                    v = __builtin_fma(-bcast_lane(quot(v), q), m, v);
+                }
+                v = quot(v);
+            } else {
+#pragma unroll
+                for (int q = 0; q < kSB; ++q) {
+                    const double m = (lane > q && i < dim) ? (double)w[q] : 0.0;
+                    v = __builtin_fma(-bcast_lane(v, q), m, v);
+                }
+            }
+            if (i < dim) sh.xs[i] = v;
         }
         __syncthreads();
-        if (wave < sb) {
-            double s = 0.0;
+        if (!chain && t < nsb && (kUpper ? st > t : st < t)) {
+            const int slot = kUpper ? st - 1 : st;
+            const int cb = kSB * st + kTC * c;
+            // four entries at a time: the scheduler must not convert all 16
+            // up front (that would spill in the upper solve)
 #pragma unroll
-            for (int c = 0; c < kSB; ++c) s = __builtin_fma(c0 + c < dim ? (double)b[c] : 0.0, xs[c0 + c], s);
-            xs[i] -= s;
-            if (sb - 1 > wave) load_row64(A, lda, d0, dim, i, c0 - kSB, b);  // next stage's sub-block
+            for (int u = 0; u < kTC; ++u) {
+                acc = __builtin_fma((double)w[kTC * slot + u], sh.xs[cb + u], acc);
+                if ((u & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+            }
+            if (st == (kUpper ? t + 1 : t - 1)) sh.pp[t][c][lane] = acc;
         }
         __syncthreads();
     }
@@ -308,61 +323,55 @@ __device__ __forceinline__ void diag_solve_upper(const float *__restrict__ A, in
 // One block column of the forward solve.  j0 = first column of the solved
 // block J (j0 < 0: nothing solved yet, only the first diagonal block).
 template <bool kVec>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kTT)
 k_trsv_lower_step(int64_t n, int64_t j0, const float *__restrict__ A, int64_t lda, double *__restrict__ x) {
-    __shared__ double xv[kTB];
-    __shared__ double xs[kTB];
-    __shared__ double part[4][kTB];
+    __shared__ TrsvShared sh;
     const int tid = threadIdx.x;
     const int64_t jb = j0 < 0 ? 0 : j0;
     const int ncols = j0 < 0 ? 0 : (int)min((int64_t)kTB, n - j0);
     const int64_t d0 = j0 < 0 ? 0 : j0 + kTB;
-    if (ncols > 0) xv[tid] = tid < ncols ? x[jb + tid] : 0.0;
+    if (ncols > 0 && tid < kTB) sh.xv[tid] = tid < ncols ? x[jb + tid] : 0.0;
     __syncthreads();
     if (blockIdx.x == 0) {
         const int dim = (int)min((int64_t)kTB, n - d0);
-        double r = (tid < dim) ? x[d0 + tid] : 0.0;
-        if (ncols > 0) r -= block_rows_dot<kVec>(A, lda, d0, d0 + dim, jb, ncols, xv, part);
-        xs[tid] = r;
+        const double t = ncols > 0 ? rows_dot<kVec>(A, lda, d0, d0 + dim, jb, ncols, sh) : 0.0;
+        if (tid < kTB) sh.xs[tid] = (tid < dim) ? x[d0 + tid] - t : 0.0;
         __syncthreads();
-        diag_solve_lower(A, lda, d0, dim, xs);
-        if (tid < dim) x[d0 + tid] = xs[tid];
+        diag_solve<false>(A, lda, d0, dim, sh);
+        if (tid < dim) x[d0 + tid] = sh.xs[tid];
         return;
     }
     const int64_t r0 = d0 + (int64_t)blockIdx.x * kTB;
-    const double t = block_rows_dot<kVec>(A, lda, r0, n, jb, ncols, xv, part);
-    if (r0 + tid < n) x[r0 + tid] -= t;
+    const double t = rows_dot<kVec>(A, lda, r0, n, jb, ncols, sh);
+    if (tid < kTB && r0 + tid < n) x[r0 + tid] -= t;
 }
 
 // One block column of the backward solve: J = [j0, j0 + ncols) solved
 // (j0 >= n: nothing solved yet, only the last diagonal block, which may be
 // shorter than kTB); D = the block before J; rows [0, d0) updated.
 template <bool kVec>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kTT)
 k_trsv_upper_step(int64_t n, int64_t j0, const float *__restrict__ A, int64_t lda, double *__restrict__ x) {
-    __shared__ double xv[kTB];
-    __shared__ double xs[kTB];
-    __shared__ double part[4][kTB];
+    __shared__ TrsvShared sh;
     const int tid = threadIdx.x;
     const int64_t last0 = ((n - 1) / kTB) * kTB;  // first row of the last block
     const bool none = j0 >= n;
     const int ncols = none ? 0 : (int)min((int64_t)kTB, n - j0);
     const int64_t d0 = none ? last0 : j0 - kTB;
     const int dim = (int)min((int64_t)kTB, n - d0);
-    if (ncols > 0) xv[tid] = tid < ncols ? x[j0 + tid] : 0.0;
+    if (ncols > 0 && tid < kTB) sh.xv[tid] = tid < ncols ? x[j0 + tid] : 0.0;
     __syncthreads();
     if (blockIdx.x == 0) {
-        double r = (tid < dim) ? x[d0 + tid] : 0.0;
-        if (ncols > 0) r -= block_rows_dot<kVec>(A, lda, d0, d0 + dim, j0, ncols, xv, part);
-        xs[tid] = r;
+        const double t = ncols > 0 ? rows_dot<kVec>(A, lda, d0, d0 + dim, j0, ncols, sh) : 0.0;
+        if (tid < kTB) sh.xs[tid] = (tid < dim) ? x[d0 + tid] - t : 0.0;
         __syncthreads();
-        diag_solve_upper(A, lda, d0, dim, xs);
-        if (tid < dim) x[d0 + tid] = xs[tid];
+        diag_solve<true>(A, lda, d0, dim, sh);
+        if (tid < dim) x[d0 + tid] = sh.xs[tid];
         return;
     }
     const int64_t r0 = (int64_t)(blockIdx.x - 1) * kTB;
-    const double t = block_rows_dot<kVec>(A, lda, r0, d0, j0, ncols, xv, part);
-    if (r0 + tid < d0) x[r0 + tid] -= t;
+    const double t = rows_dot<kVec>(A, lda, r0, d0, j0, ncols, sh);
+    if (tid < kTB && r0 + tid < d0) x[r0 + tid] -= t;
 }
 
 int grid_for(int64_t n, int threads) {
@@ -460,9 +469,9 @@ int omx_trsv_lower_unit_f32f64(int64_t n, int64_t b, const float *A, int64_t lda
         const int64_t beyond = n - d0 - kTB;
         const unsigned grid = 1u + (unsigned)(beyond > 0 && j0 >= 0 ? omx_cdiv(beyond, kTB) : 0);
         if (vec)
-            hipLaunchKernelGGL(k_trsv_lower_step<true>, dim3(grid), dim3(256), 0, st, n, j0, A, lda, x);
+            hipLaunchKernelGGL(k_trsv_lower_step<true>, dim3(grid), dim3(kTT), 0, st, n, j0, A, lda, x);
         else
-            hipLaunchKernelGGL(k_trsv_lower_step<false>, dim3(grid), dim3(256), 0, st, n, j0, A, lda, x);
+            hipLaunchKernelGGL(k_trsv_lower_step<false>, dim3(grid), dim3(kTT), 0, st, n, j0, A, lda, x);
         OMX_LAUNCH_CHECK();
     }
     return OMX_OK;
@@ -483,9 +492,9 @@ int omx_trsv_upper_f32f64(int64_t n, int64_t b, const float *A, int64_t lda, dou
         if (d0 < 0) break;
         const unsigned grid = 1u + (unsigned)(j0 < n ? omx_cdiv(d0, kTB) : 0);
         if (vec)
-            hipLaunchKernelGGL(k_trsv_upper_step<true>, dim3(grid), dim3(256), 0, st, n, j0, A, lda, x);
+            hipLaunchKernelGGL(k_trsv_upper_step<true>, dim3(grid), dim3(kTT), 0, st, n, j0, A, lda, x);
         else
-            hipLaunchKernelGGL(k_trsv_upper_step<false>, dim3(grid), dim3(256), 0, st, n, j0, A, lda, x);
+            hipLaunchKernelGGL(k_trsv_upper_step<false>, dim3(grid), dim3(kTT), 0, st, n, j0, A, lda, x);
         OMX_LAUNCH_CHECK();
     }
     return OMX_OK;
