@@ -78,7 +78,7 @@ struct omx_solver {
     DevBuf pol, feat, mlp_h, zdev;
     int has_policy = 0;
     hipEvent_t ev_dec = nullptr;
-    hipEvent_t ev_la = nullptr;        // look-ahead products of the current group done
+    std::vector<hipEvent_t> ev_la;     // per block of the next group: its look-ahead products done
     float *zhost = nullptr;            // pinned: the step's 3 outputs
     std::vector<int> last_sched;       // per step: 0 FP16, 1 + headroom FP8
     std::vector<int> prec;             // the current factorisation's choices
@@ -204,7 +204,9 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
     if (hipStreamCreateWithFlags(&s->crit, hipStreamNonBlocking) != hipSuccess) return fail(OMX_ERR_HIP);
     if (hipStreamCreateWithFlags(&s->bulk, hipStreamNonBlocking) != hipSuccess) return fail(OMX_ERR_HIP);
     const int64_t nb = s->nb;
-    if (hipEventCreateWithFlags(&s->ev_la, hipEventDisableTiming) != hipSuccess) return fail(OMX_ERR_HIP);
+    s->ev_la.resize(s->G);
+    for (auto &e : s->ev_la)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(OMX_ERR_HIP);
     s->ev_panel.resize(nb);
     s->ev_t0.resize(nb);
     s->ev_t1.resize(nb);
@@ -264,7 +266,7 @@ int omx_solver_destroy(omx_solver *s) {
     if (!s) return OMX_OK;
     if (s->crit) (void)hipStreamSynchronize(s->crit);
     if (s->bulk) (void)hipStreamSynchronize(s->bulk);
-    for (auto *v : {&s->ev_panel, &s->ev_t0, &s->ev_t1})
+    for (auto *v : {&s->ev_panel, &s->ev_t0, &s->ev_t1, &s->ev_la})
         for (auto e : *v)
             if (e) (void)hipEventDestroy(e);
     DevBuf *bufs[] = {&s->Lp[0], &s->Lp[1], &s->Rp[0], &s->Rp[1], &s->Tp, &s->L8[0], &s->L8[1], &s->R8[0],
@@ -278,7 +280,6 @@ int omx_solver_destroy(omx_solver *s) {
         dfree(s->Rse[i]);
     }
     if (s->ev_dec) (void)hipEventDestroy(s->ev_dec);
-    if (s->ev_la) (void)hipEventDestroy(s->ev_la);
     if (s->zhost) (void)hipHostFree(s->zhost);
     if (s->crit) (void)hipStreamDestroy(s->crit);
     if (s->bulk) (void)hipStreamDestroy(s->bulk);
@@ -515,8 +516,10 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
         const int64_t h0 = G * (g + 1), h1 = std::min(h0 + G - 1, nb - 1);
         // ---- bulk: the next group's blocks, then the trailing update ----
         OMX_HIP_CHECK(hipStreamWaitEvent(bs, s->ev_panel[g], 0));
-        for (int64_t t = h0; t <= h1; ++t) RC(update_block(s, t, i0, i1, A, lda, bs));
-        OMX_HIP_CHECK(hipEventRecord(s->ev_la, bs));
+        for (int64_t t = h0; t <= h1; ++t) {
+            RC(update_block(s, t, i0, i1, A, lda, bs));
+            OMX_HIP_CHECK(hipEventRecord(s->ev_la[t - h0], bs));
+        }
         const int64_t f = G * (g + 2);
         if (f < nb) {
             const int64_t m = n - f * b;
@@ -527,10 +530,11 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
             ++s->bulk_count;
         }
         // ---- crit: the next group's panels ----
-        // its blocks hold all earlier groups' updates once the look-ahead
-        // products are done
-        OMX_HIP_CHECK(hipStreamWaitEvent(cs, s->ev_la, 0));
+        // block t holds all earlier groups' updates once its look-ahead
+        // products are done; the later blocks' products still run on the
+        // bulk stream while block t is factored
         for (int64_t t = h0; t <= h1; ++t) {
+            OMX_HIP_CHECK(hipStreamWaitEvent(cs, s->ev_la[t - h0], 0));
             if (t > h0) RC(update_block(s, t, h0, t - 1, A, lda, cs));
             RC(factor(t));
         }
