@@ -26,67 +26,72 @@ __device__ __forceinline__ float apply_mask(float v, int64_t r, int64_t k, int m
     }
 }
 
-// Stage the 256 x 64 source tile (rows r0.., columns k0..) in LDS as
-// tile[k][r]; loads are coalesced along whichever dimension is contiguous.
-__device__ __forceinline__ void load_tile(float (*tile)[OMX_STRIP + 1], const float *__restrict__ src,
-                                          int64_t rs, int64_t ls, int64_t R, int64_t r0, int64_t k0,
-                                          int mask) {
-    const int t = threadIdx.x;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// n consecutive-k values (n = 8 or 16, k from kb) of source row r (clamped
+// to R - 1 so that every load is unconditional; the caller zeroes rows past
+// R): the column panel (rs = 1) by n loads, which 32 lanes with consecutive
+// rows turn into 128-byte runs, the row panel (ls = 1) by n/4 16-byte loads
+// (the host checks 16-byte alignment of such a source).
+template <int n>
+__device__ __forceinline__ void load_run(const float *__restrict__ src, int64_t rs, int64_t ls, int64_t R, int64_t r,
+                                         int64_t kb, float (&v)[n]) {
+    const int64_t rc = r < R ? r : R - 1;
     if (rs == 1) {
-        const int64_t r = r0 + t;
-#pragma unroll 8
-        for (int k = 0; k < kTileK; ++k) {
-            float v = (r < R) ? src[r + (k0 + k) * ls] : 0.0f;
-            tile[k][t] = (r < R) ? apply_mask(v, r, k0 + k, mask) : 0.0f;
-        }
+#pragma unroll
+        for (int e = 0; e < n; ++e) v[e] = src[rc + (kb + e) * ls];
     } else {
-        // contiguous along k: 16 float4 per row, 4 rows per wave per step
-#pragma unroll 4
-        for (int q = 0; q < 16; ++q) {
-            const int idx = t + kPackThreads * q;
-            const int rl = idx >> 4, k4 = (idx & 15) * 4;
-            const int64_t r = r0 + rl;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (r < R) v = *reinterpret_cast<const float4 *>(src + r * rs + (k0 + k4));
-            tile[k4 + 0][rl] = (r < R) ? apply_mask(v.x, r, k0 + k4 + 0, mask) : 0.0f;
-            tile[k4 + 1][rl] = (r < R) ? apply_mask(v.y, r, k0 + k4 + 1, mask) : 0.0f;
-            tile[k4 + 2][rl] = (r < R) ? apply_mask(v.z, r, k0 + k4 + 2, mask) : 0.0f;
-            tile[k4 + 3][rl] = (r < R) ? apply_mask(v.w, r, k0 + k4 + 3, mask) : 0.0f;
+        const float *p = src + rc * rs + kb;
+#pragma unroll
+        for (int e = 0; e < n; e += 4) {
+            const f32x4 w = *reinterpret_cast<const f32x4 *>(p + e);
+            v[e] = w.x;
+            v[e + 1] = w.y;
+            v[e + 2] = w.z;
+            v[e + 3] = w.w;
         }
     }
 }
 
 // One workgroup per (row strip, 64-column stage): 32 KB of packed output.
+// Each thread writes 8 of the stage's 2048 16-byte units (8 consecutive k of
+// one row); all 64 of its source values are loaded, straight from the panel,
+// before the first is converted, so a workgroup's 64 KB arrive in one round
+// trip.
 __global__ void __launch_bounds__(kPackThreads)
 k_pack_f16(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, int64_t ls,
            int mask, _Float16 *__restrict__ P, float *__restrict__ chunk_max,
            const int32_t *__restrict__ sexp, int64_t pchunks, int64_t pk0) {
-    __shared__ float tile[kTileK][OMX_STRIP + 1];
     __shared__ float red[kPackThreads / 64];
     const int64_t ks = blockIdx.x;   // K-stage
     const int64_t rsb = blockIdx.y;  // row strip
     const int64_t r0 = rsb * OMX_STRIP, k0 = ks * kTileK;
-    load_tile(tile, src, rs, ls, R, r0, k0, mask);
-    __syncthreads();
     // the packed panel has pchunks 64-column chunks per strip; this panel's
     // columns start at chunk pk0
     _Float16 *out = P + (rsb * pchunks + pk0 + ks) * 16384;
     const int t = threadIdx.x;
     const float scale = sexp ? ldexpf(1.0f, sexp[rsb]) : 1.0f;
-    float mx = 0.0f;
+    float v[8][8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int oidx = t + kPackThreads * q;  // 16-byte unit, 0..2047
         const int fb = oidx >> 6, lane = oidx & 63;
-        const int rl = (fb >> 2) * 32 + (lane & 31);
-        const int kb = (fb & 3) * 16 + (lane >> 5) * 8;
+        load_run<8>(src, rs, ls, R, r0 + (fb >> 2) * 32 + (lane & 31), k0 + (fb & 3) * 16 + (lane >> 5) * 8, v[q]);
+    }
+    float mx = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int oidx = t + kPackThreads * q;
+        const int fb = oidx >> 6, lane = oidx & 63;
+        const int64_t r = r0 + (fb >> 2) * 32 + (lane & 31);
+        const int64_t kb = k0 + (fb & 3) * 16 + (lane >> 5) * 8;
         typedef _Float16 half8 __attribute__((ext_vector_type(8)));
         half8 h;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const float v = tile[kb + e][rl];
-            mx = fmaxf(mx, fabsf(v));
-            h[e] = (_Float16)(v * scale);
+            const float x = (r < R) ? apply_mask(v[q][e], r, kb + e, mask) : 0.0f;
+            mx = fmaxf(mx, fabsf(x));
+            h[e] = (_Float16)(x * scale);
         }
         *reinterpret_cast<half8 *>(out + oidx * 8) = h;
     }
@@ -99,7 +104,9 @@ k_pack_f16(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, int6
     }
 }
 
-// Max |x| of each 256 x 256 (rows x K) scale tile.
+// Max |x| of each 256 x 256 (rows x K) scale tile.  Full tiles of a 16-byte
+// aligned source: each thread takes 4 consecutive rows (column panel) or
+// columns (row panel) of every fourth column (row), 16 loads in flight.
 __global__ void __launch_bounds__(kPackThreads)
 k_tile_maxabs(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, int64_t ls,
               float *__restrict__ tmax) {
@@ -108,14 +115,28 @@ k_tile_maxabs(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, i
     const int64_t r0 = rsb * OMX_STRIP, k0 = kt * OMX_SCALE_TILE;
     const int t = threadIdx.x;
     float mx = 0.0f;
-    if (rs == 1) {
+    const int64_t step = (rs == 1) ? ls : rs;  // between the lines a thread walks
+    const bool vec = (reinterpret_cast<uintptr_t>(src) & 15) == 0 && (step & 3) == 0 && r0 + OMX_STRIP <= R &&
+                     k0 + OMX_SCALE_TILE <= K;
+    if (vec) {
+        const float *p = (rs == 1) ? src + r0 + 4 * (t & 63) + (k0 + (t >> 6)) * ls
+                                   : src + (r0 + (t >> 6)) * rs + k0 + 4 * (t & 63);
+#pragma unroll 16
+        for (int j = 0; j < 64; ++j) {
+            const f32x4 w = *reinterpret_cast<const f32x4 *>(p + (int64_t)(4 * j) * step);
+            mx = fmaxf(mx, fmaxf(fmaxf(fabsf(w.x), fabsf(w.y)), fmaxf(fabsf(w.z), fabsf(w.w))));
+        }
+    } else if (rs == 1) {
         const int64_t r = r0 + t;
-        if (r < R)
+        if (r < R) {
+#pragma unroll 8
             for (int64_t k = k0; k < k0 + OMX_SCALE_TILE && k < K; ++k) mx = fmaxf(mx, fabsf(src[r + k * ls]));
+        }
     } else {
+        const int64_t k = k0 + t;
+#pragma unroll 8
         for (int q = 0; q < OMX_STRIP; ++q) {
             const int64_t r = r0 + q;
-            const int64_t k = k0 + t;
             if (r < R && k < K) mx = fmaxf(mx, fabsf(src[r * rs + k]));
         }
     }
@@ -163,37 +184,41 @@ __global__ void k_strip_exps16(int64_t nstrips, int64_t KT, const float *__restr
     sexp[s] = e;
 }
 
-// FP8 pack: one workgroup per (row strip, 64-column half of a 128 stage).
+// FP8 pack: one workgroup per (row strip, 64-column half of a 128 stage);
+// each thread writes 4 16-byte units (16 consecutive k of one row), its 64
+// source values loaded straight from the panel first.
 __global__ void __launch_bounds__(kPackThreads)
 k_pack_f8(int64_t R, int64_t K, const float *__restrict__ src, int64_t rs, int64_t ls,
           const int32_t *__restrict__ exps, uint8_t *__restrict__ P, int64_t pchunks, int64_t pk0,
           int64_t ekt, int64_t ek0) {
-    __shared__ float tile[kTileK][OMX_STRIP + 1];
     const int64_t k64 = blockIdx.x;  // 64-column block
     const int64_t rsb = blockIdx.y;
     const int64_t r0 = rsb * OMX_STRIP, k0 = k64 * kTileK;
-    load_tile(tile, src, rs, ls, R, r0, k0, OMX_MASK_NONE);
-    __syncthreads();
     const float scale = ldexpf(1.0f, exps[rsb * ekt + ek0 + k0 / OMX_SCALE_TILE]);
     const int64_t ks = k64 >> 1;
     const int kk = (int)(k64 & 1);
     // pchunks 128-column chunks per strip; this panel starts at chunk pk0
     uint8_t *chunk = P + (rsb * pchunks + pk0 + ks) * 32768;
     const int t = threadIdx.x;
+    float v[4][16];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int oidx = t + kPackThreads * q;  // 0..1023: (rr, h, lane)
         const int rr = oidx >> 7, h = (oidx >> 6) & 1, lane = oidx & 63;
-        const int rl = rr * 32 + (lane & 31);
-        const int kb = (lane >> 5) * 32 + h * 16;  // within the 64-column block
+        load_run<16>(src, rs, ls, R, r0 + rr * 32 + (lane & 31), k0 + (lane >> 5) * 32 + h * 16, v[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int oidx = t + kPackThreads * q;
+        const int rr = oidx >> 7, h = (oidx >> 6) & 1, lane = oidx & 63;
+        const bool in = r0 + rr * 32 + (lane & 31) < R;
         uint32_t w[4];
 #pragma unroll
         for (int e4 = 0; e4 < 4; ++e4) {
+            const float *x = &v[q][4 * e4];
             int pk = 0;
-            pk = __builtin_amdgcn_cvt_pk_fp8_f32(tile[kb + 4 * e4 + 0][rl] * scale,
-                                                 tile[kb + 4 * e4 + 1][rl] * scale, pk, false);
-            pk = __builtin_amdgcn_cvt_pk_fp8_f32(tile[kb + 4 * e4 + 2][rl] * scale,
-                                                 tile[kb + 4 * e4 + 3][rl] * scale, pk, true);
+            pk = __builtin_amdgcn_cvt_pk_fp8_f32(in ? x[0] * scale : 0.0f, in ? x[1] * scale : 0.0f, pk, false);
+            pk = __builtin_amdgcn_cvt_pk_fp8_f32(in ? x[2] * scale : 0.0f, in ? x[3] * scale : 0.0f, pk, true);
             w[e4] = (uint32_t)pk;
         }
         uint8_t *dst = chunk + ((rr * 2 + kk) * 2048) + h * 1024 + lane * 16;
