@@ -436,6 +436,7 @@ typedef struct omx_solver omx_solver;
 #define OMX_SOLVER_FP8 1     /* flag of omx_solver_create: also allocate FP8 panels */
 #define OMX_SOLVER_SCALE16 2 /* flag: FP16 Schur panels with per-strip pow2 scales */
 #define OMX_SOLVER_GROUP4 4  /* flag: one trailing update per 4 panels (depth 4b) instead of per 2 */
+#define OMX_SOLVER_TIMERS 8  /* flag: phase timers (HIP events around every timed phase) */
 
 /* Create a solver for order n, block b (n % b == 0, b % 256 == 0), panel
  * method alt (0: FP32 TRSM, 1: FP16 GEMM with the TRSM-on-identity inverse,
@@ -470,6 +471,25 @@ int omx_solver_set_policy(omx_solver *s, const float *params);
  * (features, n/b x 16 floats, row-major; zero for the last step, which has
  * no panels).  Either pointer may be NULL. */
 int omx_solver_last_schedule(omx_solver *s, int *sched, float *features);
+
+/* Phase timers of a solver created with OMX_SOLVER_TIMERS: GPU seconds and
+ * operation counts per phase of the last omx_lu_factor and the refinement
+ * calls since it (omx_lu_factor resets them, as the reference's
+ * Timer::initialize at the start of a run), numbered and named as the
+ * reference's Timer items (timer.hpp:57-81, names :120-145; replaces
+ * Timer::show / dump_mp, :160-220).  One GPU has no broadcasts, so the
+ * *_BCAST, *_COMM, TEST and WAIT items stay 0.  Timed: DIAG_LU (the
+ * diagonal block's LU and -alt inverses), TRSM_L / TRSM_R (the column /
+ * row panel: FP32 TRSM or -alt inverse product), CONV_L / CONV_R (their
+ * FP16/FP8 packs), GEMM_UPDATE (every Schur product: trailing, look-ahead
+ * and within a group), IR_GEMV, IR_TRSV; TOTAL is the wall time of those
+ * calls.  Phases on the two streams overlap, so the items can sum to more
+ * than TOTAL.  seconds, ops: host arrays of OMX_TIMER_ITEMS doubles.
+ * Returns OMX_ERR_ARG when the solver has no timers. */
+#define OMX_TIMER_ITEMS 23
+int omx_solver_timers(const omx_solver *s, double *seconds, double *ops);
+/* Name of timer item i (0 <= i < OMX_TIMER_ITEMS), or NULL. */
+const char *omx_timer_name(int item);
 
 /* FP64 refinement of A x = rhs with the FP32 factors LU (device) and the
  * generated A (regenerated from seed; diag = its FP64 diagonal).

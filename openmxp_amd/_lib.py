@@ -76,6 +76,8 @@ SIGNATURES = {
     "omx_lu_factor": (i32, [vp, vp, i64, vp, vp]),
     "omx_solver_set_policy": (i32, [vp, vp]),
     "omx_solver_last_schedule": (i32, [vp, vp, vp]),
+    "omx_solver_timers": (i32, [vp, vp, vp]),
+    "omx_timer_name": (C.c_char_p, [i32]),
     "omx_ir_solve": (i32, [vp, vp, i64, u64, vp, vp, vp, i32, i32, i32, dbl, dbl, i32, vp, vp]),
     "omx_ir_solve_system": (i32, [vp, vp, i64, vp, vp, vp, i32, i32, i32, dbl, dbl, i32, vp, vp]),
     "omx_harness": (dbl, [dbl, dbl, dbl, dbl, i64]),
@@ -92,6 +94,8 @@ class System(C.Structure):
 SOLVER_FP8 = 1      # omx_solver_create flags
 SOLVER_SCALE16 = 2
 SOLVER_GROUP4 = 4
+SOLVER_TIMERS = 8
+TIMER_ITEMS = 23
 
 SYSTEM_LCG = 0
 SYSTEM_LOWRANK = 1

@@ -39,6 +39,12 @@
 #include "omx_common.hpp"
 #include "omx_layout.hpp"
 
+#define RC(x)                  \
+    do {                       \
+        int rc_ = (x);         \
+        if (rc_) return rc_;   \
+    } while (0)
+
 namespace {
 
 struct DevBuf {
@@ -56,6 +62,93 @@ void dfree(DevBuf &b) {
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
 }
+
+// Phase timers (OMX_SOLVER_TIMERS): each timed phase records a pair of
+// events from a reusable pool on the stream that runs it; after the call
+// synchronises, the pairs' elapsed times are summed per item.  Items follow
+// the reference's Timer enumeration (timer.hpp:57-81).
+struct PhaseTimers {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    struct Span {
+        int item;
+        size_t e0, e1;
+        double ops;
+    };
+    std::vector<Span> spans;
+    double seconds[OMX_TIMER_ITEMS] = {}, ops[OMX_TIMER_ITEMS] = {};
+
+    int event(size_t &idx) {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return OMX_ERR_HIP;
+            pool.push_back(e);
+        }
+        idx = used++;
+        return OMX_OK;
+    }
+    // start a phase on st; returns the span index through *span (-1 when off)
+    int beg(int item, hipStream_t st, long *span) {
+        *span = -1;
+        if (!on) return OMX_OK;
+        size_t e0;
+        RC(event(e0));
+        if (hipEventRecord(pool[e0], st) != hipSuccess) return OMX_ERR_HIP;
+        spans.push_back(Span{item, e0, e0, 0.0});
+        *span = (long)spans.size() - 1;
+        return OMX_OK;
+    }
+    int end(long span, hipStream_t st, double op) {
+        if (!on || span < 0) return OMX_OK;
+        size_t e1;
+        RC(event(e1));
+        if (hipEventRecord(pool[e1], st) != hipSuccess) return OMX_ERR_HIP;
+        spans[span].e1 = e1;
+        spans[span].ops = op;
+        return OMX_OK;
+    }
+    // after the streams are synchronised
+    int collect() {
+        for (const Span &sp : spans) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, pool[sp.e0], pool[sp.e1]) != hipSuccess) return OMX_ERR_HIP;
+            seconds[sp.item] += ms * 1e-3;
+            ops[sp.item] += sp.ops;
+        }
+        spans.clear();
+        used = 0;
+        return OMX_OK;
+    }
+    void reset() {
+        for (int i = 0; i < OMX_TIMER_ITEMS; ++i) seconds[i] = ops[i] = 0.0;
+        spans.clear();
+        used = 0;
+    }
+};
+
+// Run f (returning an OMX code) as phase `item` on stream st with `op`
+// operations.
+#define TIMED(T, item, st, op, f)                  \
+    do {                                           \
+        long sp_;                                  \
+        RC((T).beg((item), (st), &sp_));           \
+        RC(f);                                     \
+        RC((T).end(sp_, (st), (op)));              \
+    } while (0)
+
+// the reference's Timer item numbers (timer.hpp:57-81) of the timed phases
+enum {
+    kT_DIAG_LU = 5,
+    kT_TRSM_L = 6,
+    kT_TRSM_R = 7,
+    kT_CONV_L = 8,
+    kT_CONV_R = 9,
+    kT_GEMM_UPDATE = 10,
+    kT_IR_GEMV = 14,
+    kT_IR_TRSV = 16,
+    kT_TOTAL = 21,
+};
 
 }  // namespace
 
@@ -90,6 +183,7 @@ struct omx_solver {
     double t_lu_ms = 0, bulk_ms = 0, bulk_flops = 0;
     int bulk_count = 0;
     double crit_flops = 0;
+    PhaseTimers timers;
 };
 
 namespace {
@@ -176,12 +270,6 @@ double wall_ms() {
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-#define RC(x)                  \
-    do {                       \
-        int rc_ = (x);         \
-        if (rc_) return rc_;   \
-    } while (0)
-
 }  // namespace
 
 extern "C" {
@@ -196,6 +284,7 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
     s->fp8 = (flags & OMX_SOLVER_FP8) ? 1 : 0;
     s->scale16 = (flags & OMX_SOLVER_SCALE16) ? 1 : 0;
     s->G = (flags & OMX_SOLVER_GROUP4) ? 4 : 2;
+    s->timers.on = (flags & OMX_SOLVER_TIMERS) != 0;
     int rc = OMX_OK;
     auto fail = [&](int code) {
         omx_solver_destroy(s);
@@ -280,6 +369,7 @@ int omx_solver_destroy(omx_solver *s) {
         dfree(s->Rse[i]);
     }
     if (s->ev_dec) (void)hipEventDestroy(s->ev_dec);
+    for (auto e : s->timers.pool) (void)hipEventDestroy(e);
     if (s->zhost) (void)hipHostFree(s->zhost);
     if (s->crit) (void)hipStreamDestroy(s->crit);
     if (s->bulk) (void)hipStreamDestroy(s->bulk);
@@ -337,9 +427,13 @@ static int apply_panels(omx_solver *s, int64_t i0, int64_t i1, int64_t r0, int64
 // Block row/col t (A_tt, A[t+1:, t], A[t, t+1:]) -= panels i0 .. i1.
 static int update_block(omx_solver *s, int64_t t, int64_t i0, int64_t i1, float *A, int64_t lda, hipStream_t st) {
     const int64_t n = s->n, b = s->b, tb = t * b;
+    const double fl = 2.0 * (double)(i1 - i0 + 1) * b * b * (2.0 * (n - tb) - b);
+    long sp;
+    RC(s->timers.beg(kT_GEMM_UPDATE, st, &sp));
     RC(apply_panels(s, i0, i1, tb, tb, n - tb, b, A, lda, st));
     RC(apply_panels(s, i0, i1, tb, tb + b, b, n - tb - b, A, lda, st));
-    s->crit_flops += 2.0 * (double)(i1 - i0 + 1) * b * b * (2.0 * (n - tb) - b);
+    RC(s->timers.end(sp, st, fl));
+    s->crit_flops += fl;
     return OMX_OK;
 }
 
@@ -358,10 +452,12 @@ static int pack_panel(omx_solver *s, int64_t i, int have_max, int64_t rest, cons
         uint8_t *R8 = static_cast<uint8_t *>(s->R8[buf].p) + rstrip * sb8;
         int32_t *Le = static_cast<int32_t *>(s->Le[buf].p) + rstrip * ne;
         int32_t *Re = static_cast<int32_t *>(s->Re[buf].p) + rstrip * ne;
-        RC(omx_pack_f8_window(rest, b, Acol, 1, lda, h, have_max, L8, Le, ldk, k0, static_cast<float *>(s->L8max.p),
-                              st));
-        RC(omx_pack_f8_window(rest, b, Arow, lda, 1, h, have_max, R8, Re, ldk, k0, static_cast<float *>(s->R8max.p),
-                              st));
+        TIMED(s->timers, kT_CONV_L, st, (double)rest * b,
+              omx_pack_f8_window(rest, b, Acol, 1, lda, h, have_max, L8, Le, ldk, k0,
+                                 static_cast<float *>(s->L8max.p), st));
+        TIMED(s->timers, kT_CONV_R, st, (double)rest * b,
+              omx_pack_f8_window(rest, b, Arow, lda, 1, h, have_max, R8, Re, ldk, k0,
+                                 static_cast<float *>(s->R8max.p), st));
         return OMX_OK;
     }
     const size_t sb16 = (size_t)(ldk / OMX_KSTAGE16) * 16384;  // uint16 per strip
@@ -372,8 +468,10 @@ static int pack_panel(omx_solver *s, int64_t i, int have_max, int64_t rest, cons
         xe = static_cast<int32_t *>(s->Lse[G * buf + i % G].p) + rstrip;
         ye = static_cast<int32_t *>(s->Rse[G * buf + i % G].p) + rstrip;
     }
-    RC(omx_pack_f16_window(rest, b, Acol, 1, lda, have_max, Lp, xe, static_cast<float *>(s->L8max.p), ldk, k0, st));
-    RC(omx_pack_f16_window(rest, b, Arow, lda, 1, have_max, Rp, ye, static_cast<float *>(s->R8max.p), ldk, k0, st));
+    TIMED(s->timers, kT_CONV_L, st, (double)rest * b,
+          omx_pack_f16_window(rest, b, Acol, 1, lda, have_max, Lp, xe, static_cast<float *>(s->L8max.p), ldk, k0, st));
+    TIMED(s->timers, kT_CONV_R, st, (double)rest * b,
+          omx_pack_f16_window(rest, b, Arow, lda, 1, have_max, Rp, ye, static_cast<float *>(s->R8max.p), ldk, k0, st));
     return OMX_OK;
 }
 
@@ -456,6 +554,8 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
     s->crit_flops = 0;
     std::fill(s->prec.begin(), s->prec.end(), 0);
     OMX_HIP_CHECK(hipDeviceSynchronize());
+    s->timers.reset();
+    PhaseTimers &T = s->timers;
     const double t0 = wall_ms();
     if (stats) OMX_HIP_CHECK(hipMemsetAsync(s->feat.p, 0, s->feat.bytes, cs));
 
@@ -467,26 +567,35 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
     auto factor = [&](int64_t i) -> int {
         const int64_t ib = i * b, rest = n - ib - b;
         float *Akk = A + ib + ib * lda;
-        RC(omx_getrf_nopiv_f32(b, Akk, lda, info + i, cs));
+        TIMED(T, kT_DIAG_LU, cs, 2.0 / 3.0 * b * (double)b * b, omx_getrf_nopiv_f32(b, Akk, lda, info + i, cs));
         int choice = 0;
         if (rest > 0) {
             float *Acol = A + (ib + b) + ib * lda;
             float *Arow = A + ib + (ib + b) * lda;
+            const double rb2 = (double)rest * b * b;
             if (i == 0 || s->alt == 0) {
-                RC(omx_trsm_f32(OMX_TRSM_RUNN, rest, b, Akk, lda, Acol, lda, cs));
-                RC(omx_trsm_f32(OMX_TRSM_LLNU, b, rest, Akk, lda, Arow, lda, cs));
+                TIMED(T, kT_TRSM_L, cs, rb2, omx_trsm_f32(OMX_TRSM_RUNN, rest, b, Akk, lda, Acol, lda, cs));
+                TIMED(T, kT_TRSM_R, cs, rb2, omx_trsm_f32(OMX_TRSM_LLNU, b, rest, Akk, lda, Arow, lda, cs));
             } else {
                 // U^-1 and L^-1 of the factored diagonal block
-                RC(omx_trtri_lu_f32(b, Akk, lda, Wup, Wlo, b, static_cast<float *>(s->Wtmp.p), cs));
+                TIMED(T, kT_DIAG_LU, cs, 2.0 / 3.0 * b * (double)b * b,
+                      omx_trtri_lu_f32(b, Akk, lda, Wup, Wlo, b, static_cast<float *>(s->Wtmp.p), cs));
                 // Y(j, l) = W(l, j), upper part only for -alt 2
+                long sp;
+                RC(T.beg(kT_CONV_L, cs, &sp));
                 RC(omx_pack_f16(b, b, Wup, b, 1, s->alt == 2 ? OMX_MASK_K_LE_R : OMX_MASK_NONE, Wup_h, nullptr, cs));
                 RC(omx_pack_f16(rest, b, Acol, 1, lda, OMX_MASK_NONE, Tp, nullptr, cs));
-                RC(omx_gemm_f16_packed(rest, b, b, 1.0f, Tp, 0, Wup_h, 0, 0.0f, Acol, lda, cs));
+                RC(T.end(sp, cs, (double)(rest + b) * b));
+                TIMED(T, kT_TRSM_L, cs, 2.0 * rb2,
+                      omx_gemm_f16_packed(rest, b, b, 1.0f, Tp, 0, Wup_h, 0, 0.0f, Acol, lda, cs));
                 // X(i, l) = W(i, l), strict lower + unit diagonal for -alt 2
+                RC(T.beg(kT_CONV_R, cs, &sp));
                 RC(omx_pack_f16(b, b, Wlo, 1, b, s->alt == 2 ? OMX_MASK_R_GT_K_UNIT : OMX_MASK_NONE, Wlo_h, nullptr,
                                 cs));
                 RC(omx_pack_f16(rest, b, Arow, lda, 1, OMX_MASK_NONE, Tp, nullptr, cs));
-                RC(omx_gemm_f16_packed(b, rest, b, 1.0f, Wlo_h, 0, Tp, 0, 0.0f, Arow, lda, cs));
+                RC(T.end(sp, cs, (double)(rest + b) * b));
+                TIMED(T, kT_TRSM_R, cs, 2.0 * rb2,
+                      omx_gemm_f16_packed(b, rest, b, 1.0f, Wlo_h, 0, Tp, 0, 0.0f, Arow, lda, cs));
             }
             s->crit_flops += 4.0 * rest * (double)b * b;
             const int prev = (i > 0 && s->prec[i - 1] > 0) ? 1 : 0;
@@ -524,7 +633,8 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
         if (f < nb) {
             const int64_t m = n - f * b;
             OMX_HIP_CHECK(hipEventRecord(s->ev_t0[s->bulk_count], bs));
-            RC(apply_panels(s, i0, i1, f * b, f * b, m, m, A, lda, bs));
+            TIMED(T, kT_GEMM_UPDATE, bs, 2.0 * (double)m * m * (double)((i1 - i0 + 1) * b),
+                  apply_panels(s, i0, i1, f * b, f * b, m, m, A, lda, bs));
             OMX_HIP_CHECK(hipEventRecord(s->ev_t1[s->bulk_count], bs));
             s->bulk_flops += 2.0 * (double)m * m * (double)((i1 - i0 + 1) * b);
             ++s->bulk_count;
@@ -543,6 +653,8 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
     OMX_HIP_CHECK(hipStreamSynchronize(cs));
     OMX_HIP_CHECK(hipStreamSynchronize(bs));
     s->t_lu_ms = wall_ms() - t0;
+    RC(T.collect());
+    T.seconds[kT_TOTAL] += s->t_lu_ms * 1e-3;
     s->bulk_ms = 0;
     for (int i = 0; i < s->bulk_count; ++i) {
         float ms = 0;
@@ -593,9 +705,19 @@ static int system_gemv(omx_solver *s, const omx_system *sys, double alpha, const
                                 s->gemv_ws.bytes, st);
 }
 
-int omx_ir_solve_system(omx_solver *s, const float *LU, int64_t lda, const omx_system *sys, const double *rhs,
-                        double *x, int method, int maxit, int restart, double inner_tol, double stop_harness,
-                        int x_given, double *out, double *trace) {
+// r <- U^-1 L^-1 r with the FP32 factors (one IR_TRSV phase)
+static int trsv_pair(omx_solver *s, const float *LU, int64_t lda, double *r, hipStream_t st) {
+    const int64_t n = s->n, b = s->b;
+    long sp;
+    RC(s->timers.beg(kT_IR_TRSV, st, &sp));
+    RC(omx_trsv_lower_unit_f32f64(n, b, LU, lda, r, s->trsv_ws.p, s->trsv_ws.bytes, st));
+    RC(omx_trsv_upper_f32f64(n, b, LU, lda, r, s->trsv_ws.p, s->trsv_ws.bytes, st));
+    return s->timers.end(sp, st, 2.0 * n * (double)n);
+}
+
+static int ir_solve_impl(omx_solver *s, const float *LU, int64_t lda, const omx_system *sys, const double *rhs,
+                         double *x, int method, int maxit, int restart, double inner_tol, double stop_harness,
+                         int x_given, double *out, double *trace) {
     if (!s || !LU || !sys || !sys->diag || !rhs || !x || maxit <= 0 || lda < s->n) return OMX_ERR_ARG;
     if (sys->kind != OMX_SYSTEM_LCG &&
         (sys->kind != OMX_SYSTEM_LOWRANK || sys->rank < 0 || sys->rank > 16 || !sys->sigma ||
@@ -628,7 +750,7 @@ int omx_ir_solve_system(omx_solver *s, const float *LU, int64_t lda, const omx_s
     for (it = 0; it < maxit; ++it) {
         RC(omx_copy_f64(n, rhs, r, st));
         RC(omx_norm_inf_f64(n, x, nrm, st));
-        RC(system_gemv(s, sys, -1.0, x, r, st));
+        TIMED(s->timers, kT_IR_GEMV, st, 2.0 * n * (double)n, system_gemv(s, sys, -1.0, x, r, st));
         RC(omx_norm_inf_f64(n, r, nrm + 1, st));
         OMX_HIP_CHECK(hipMemcpyAsync(hn, nrm, 16, hipMemcpyDeviceToHost, st));
         OMX_HIP_CHECK(hipStreamSynchronize(st));
@@ -649,16 +771,14 @@ int omx_ir_solve_system(omx_solver *s, const float *LU, int64_t lda, const omx_s
             break;
         }
         if (method == 0) {
-            RC(omx_trsv_lower_unit_f32f64(n, b, LU, lda, r, s->trsv_ws.p, s->trsv_ws.bytes, st));
-            RC(omx_trsv_upper_f32f64(n, b, LU, lda, r, s->trsv_ws.p, s->trsv_ws.bytes, st));
+            RC(trsv_pair(s, LU, lda, r, st));
             RC(omx_axpy_f64(n, 1.0, r, x, st));
             continue;
         }
         // ---- GMRES(restart) on the preconditioned correction equation ----
         const int m = restart;
         double *V = static_cast<double *>(s->V.p), *hv = static_cast<double *>(s->hv.p);
-        RC(omx_trsv_lower_unit_f32f64(n, b, LU, lda, r, s->trsv_ws.p, s->trsv_ws.bytes, st));
-        RC(omx_trsv_upper_f32f64(n, b, LU, lda, r, s->trsv_ws.p, s->trsv_ws.bytes, st));
+        RC(trsv_pair(s, LU, lda, r, st));
         RC(dots(s, n, 1, r, n, r, hv, st));
         double beta = 0;
         OMX_HIP_CHECK(hipMemcpyAsync(&beta, hv, 8, hipMemcpyDeviceToHost, st));
@@ -674,9 +794,8 @@ int omx_ir_solve_system(omx_solver *s, const float *LU, int64_t lda, const omx_s
             double *vj = V + (int64_t)j * n;
             hipLaunchKernelGGL(k_zero_d, dim3(vgrid(n)), dim3(256), 0, st, n, w);
             OMX_LAUNCH_CHECK();
-            RC(system_gemv(s, sys, 1.0, vj, w, st));
-            RC(omx_trsv_lower_unit_f32f64(n, b, LU, lda, w, s->trsv_ws.p, s->trsv_ws.bytes, st));
-            RC(omx_trsv_upper_f32f64(n, b, LU, lda, w, s->trsv_ws.p, s->trsv_ws.bytes, st));
+            TIMED(s->timers, kT_IR_GEMV, st, 2.0 * n * (double)n, system_gemv(s, sys, 1.0, vj, w, st));
+            RC(trsv_pair(s, LU, lda, w, st));
             // classical Gram-Schmidt, twice
             std::fill(h.begin(), h.end(), 0.0);
             for (int pass = 0; pass < 2; ++pass) {
@@ -734,6 +853,43 @@ int omx_ir_solve_system(omx_solver *s, const float *LU, int64_t lda, const omx_s
         out[3] = inner_total;
     }
     return result;
+}
+
+int omx_ir_solve_system(omx_solver *s, const float *LU, int64_t lda, const omx_system *sys, const double *rhs,
+                        double *x, int method, int maxit, int restart, double inner_tol, double stop_harness,
+                        int x_given, double *out, double *trace) {
+    if (!s) return OMX_ERR_ARG;
+    const double t0 = wall_ms();
+    const int rc = ir_solve_impl(s, LU, lda, sys, rhs, x, method, maxit, restart, inner_tol, stop_harness, x_given,
+                                 out, trace);
+    if (s->timers.on) {
+        OMX_HIP_CHECK(hipStreamSynchronize(s->crit));
+        RC(s->timers.collect());
+        s->timers.seconds[kT_TOTAL] += (wall_ms() - t0) * 1e-3;
+    }
+    return rc;
+}
+
+static const char *const kTimerNames[OMX_TIMER_ITEMS] = {
+    "DIAG_BCAST", "LCOL_BCAST", "RROW_BCAST",   "TEST",    "WAIT",         "DIAG_LU", "TRSM_L", "TRSM_R",
+    "CONV_L",     "CONV_R",     "GEMM_UPDATE",  "GEMM_PROGRESS", "LAZY_INIT", "WRITE_BACK", "IR_GEMV",
+    "IR_GEMV_COMM", "IR_TRSV",  "IR_TRSV_MV",   "IR_TRSV_COMM", "INIT",  "MEMCPY",  "TOTAL",  "MISC"};
+
+int omx_solver_timers(const omx_solver *s, double *seconds, double *ops) {
+    if (!s || !s->timers.on || !seconds || !ops) return OMX_ERR_ARG;
+    double sum = 0.0;
+    for (int i = 0; i < OMX_TIMER_ITEMS; ++i) {
+        seconds[i] = s->timers.seconds[i];
+        ops[i] = s->timers.ops[i];
+        if (i < kT_TOTAL) sum += seconds[i];
+    }
+    // as Timer::show (timer.hpp:166-172): MISC = TOTAL - the other items
+    seconds[OMX_TIMER_ITEMS - 1] = seconds[kT_TOTAL] - sum;
+    return OMX_OK;
+}
+
+const char *omx_timer_name(int item) {
+    return (item >= 0 && item < OMX_TIMER_ITEMS) ? kTimerNames[item] : nullptr;
 }
 
 }  // extern "C"

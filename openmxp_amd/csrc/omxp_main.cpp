@@ -29,6 +29,7 @@ struct Opts {
     int solv = 1, alt = 1, log = 1, progress = 10, init = 1, skip = 0;
     int ir = 0, maxit = 50, restart = 50, device = 0;
     int group = 0;  // panels per trailing update: 0 = 4 for fp8/learned, 2 for fp16
+    int timer = 0;  // print the phase timers (Timer::show, timer.hpp:160-182) after the run
     double inner_tol = 1e-10;
     unsigned long long seed = 42;
     std::string prec = "fp16";
@@ -39,7 +40,7 @@ void usage() {
     std::printf(
         "usage: omxp N B [P Q] [-solv 0|1] [-alt 0|1|2] [-log 0|1] [-progress k] [-init 1]\n"
         "            [-prec fp16|fp8|learned] [-policy file] [-ir classic|gmres] [-seed s] [-maxit k]\n"
-        "            [-restart m] [-device d] [-group 2|4]\n");
+        "            [-restart m] [-device d] [-group 2|4] [-timer]\n");
 }
 
 bool parse(int argc, char **argv, Opts &o) {
@@ -74,6 +75,7 @@ bool parse(int argc, char **argv, Opts &o) {
         else if (!std::strcmp(a, "-restart")) o.restart = std::atoi(next(a));
         else if (!std::strcmp(a, "-device")) o.device = std::atoi(next(a));
         else if (!std::strcmp(a, "-group")) o.group = std::atoi(next(a));
+        else if (!std::strcmp(a, "-timer")) o.timer = 1;
         else if (!std::strcmp(a, "-h") || !std::strcmp(a, "--help")) {
             usage();
             std::exit(0);
@@ -165,7 +167,9 @@ int main(int argc, char **argv) {
     CK(hipMalloc(&x, sizeof(double) * n));
     omx_solver *s = nullptr;
     const int group = o.group ? o.group : (fp8 ? 4 : 2);
-    if (omx_solver_create(n, b, o.alt, (fp8 ? OMX_SOLVER_FP8 : 0) | (group == 4 ? OMX_SOLVER_GROUP4 : 0), &s) != 0) {
+    const int flags =
+        (fp8 ? OMX_SOLVER_FP8 : 0) | (group == 4 ? OMX_SOLVER_GROUP4 : 0) | (o.timer ? OMX_SOLVER_TIMERS : 0);
+    if (omx_solver_create(n, b, o.alt, flags, &s) != 0) {
         std::fprintf(stderr, "solver creation failed\n");
         return 1;
     }
@@ -229,6 +233,20 @@ int main(int argc, char **argv) {
     std::printf("PFACT  : Right\nDEPTH  : 1\nPREC   : %s\n", o.prec.c_str());
     std::printf("TIME(s): %.3f\nGFLOPS : %.1f\nRESID  : %20.15e\nHARNESS: %.9f\n", dur, flops / dur * 1e-9, resid,
                 harness);
+    if (o.timer) {
+        // the reference's Timer::show lines (timer.hpp:160-182), GPU seconds per phase
+        double sec[OMX_TIMER_ITEMS], ops[OMX_TIMER_ITEMS];
+        if (omx_solver_timers(s, sec, ops) == 0) {
+            const double total = sec[OMX_TIMER_ITEMS - 2];
+            for (int i = 0; i < OMX_TIMER_ITEMS; ++i)
+                std::printf(" %-12s : %e sec : %6.2f %% : %20lld : %e Gop/s\n", omx_timer_name(i), sec[i],
+                            total > 0 ? 100.0 * sec[i] / total : 0.0, (long long)ops[i],
+                            sec[i] > 0 ? 1e-9 * ops[i] / sec[i] : 0.0);
+            const int gu = 10, gp = 11;  // GEMM_UPDATE, GEMM_PROGRESS
+            const double gt = sec[gu] + sec[gp];
+            std::printf("GEMM_TOTAL: %f Tflops\n", gt > 0 ? 1e-12 * (ops[gu] + ops[gp]) / gt : 0.0);
+        }
+    }
     omx_solver_destroy(s);
     (void)hipFree(A);
     (void)hipFree(d);

@@ -47,3 +47,24 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(_lib.OmxError):
         _lib.lib()
+
+
+# the reference's Timer item names in enumeration order (timer.hpp:57-81,
+# :120-145)
+REFERENCE_TIMER_ITEMS = [
+    "DIAG_BCAST", "LCOL_BCAST", "RROW_BCAST", "TEST", "WAIT", "DIAG_LU", "TRSM_L", "TRSM_R", "CONV_L", "CONV_R",
+    "GEMM_UPDATE", "GEMM_PROGRESS", "LAZY_INIT", "WRITE_BACK", "IR_GEMV", "IR_GEMV_COMM", "IR_TRSV", "IR_TRSV_MV",
+    "IR_TRSV_COMM", "INIT", "MEMCPY", "TOTAL", "MISC"]
+
+
+def test_timer_items_are_the_references():
+    from openmxp_amd import _lib
+    L = _lib.lib()
+    assert _lib.TIMER_ITEMS == len(REFERENCE_TIMER_ITEMS)
+    got = [L.omx_timer_name(i).decode() for i in range(_lib.TIMER_ITEMS)]
+    assert got == REFERENCE_TIMER_ITEMS
+    assert L.omx_timer_name(-1) is None and L.omx_timer_name(_lib.TIMER_ITEMS) is None
+    # no solver, no timers
+    buf = (ctypes.c_double * _lib.TIMER_ITEMS)()
+    assert L.omx_solver_timers(None, buf, buf) < 0
+

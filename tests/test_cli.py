@@ -63,3 +63,20 @@ def test_learned_mode_with_a_constant_policy(tmp_path):
     assert r.returncode == 0, r.stderr
     assert "learned schedule: 7 of 7 steps in FP8" in r.stdout
     assert _harness(r.stdout) < 16.0
+
+
+@pytest.mark.gpu
+def test_timer_report():
+    """-timer prints the reference's Timer::show lines (timer.hpp:160-182):
+    every item, then the GEMM_TOTAL rate."""
+    from tests.gpu_util import torch_cuda
+    torch_cuda()
+    r = run(2048, 256, "-timer")
+    print(r.stdout[-1500:], flush=True)
+    assert r.returncode == 0, r.stderr
+    lines = [l for l in r.stdout.splitlines() if " sec : " in l]
+    assert len(lines) == 23
+    items = {l.split(":")[0].strip(): float(l.split(":")[1].split()[0]) for l in lines}
+    assert items["GEMM_UPDATE"] > 0 and items["DIAG_LU"] > 0 and items["IR_TRSV"] > 0 and items["TOTAL"] > 0
+    assert any(l.startswith("GEMM_TOTAL:") and float(l.split()[1]) > 0 for l in r.stdout.splitlines())
+

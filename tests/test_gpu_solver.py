@@ -267,3 +267,60 @@ def test_lu_group4_mixed_precision_schedule():
     d2 = np.abs(facs[0] - facs[1])
     assert d2[low].max() <= 2.0**-4 * np.abs(facs[0][low]).max()
     assert np.max(d2[~low] / (np.abs(facs[0][~low]) + 0.05)) <= 2.0**-4
+
+
+@pytest.mark.parametrize("alt,method", [(1, 0), (1, 1), (0, 0)])
+def test_phase_timers(alt, method):
+    """With OMX_SOLVER_TIMERS the solver reports GPU time per phase under the
+    reference's Timer items; the operation counts are exact: the Schur
+    products sum to sum_k 2 b (n - (k+1) b)^2 whatever the grouping, the
+    diagonal LUs (and -alt inverses) to 2/3 b^3 each, each residual to 2 n^2,
+    each pair of triangular solves to 2 n^2."""
+    from openmxp_amd import _lib
+    torch = torch_cuda()
+    n, b, seed = 2048, 256, 42
+    print(f"phase timers alt={alt} method={method}", flush=True)
+    L = _L()
+    h = C.c_void_p()
+    assert L.omx_solver_create(n, b, alt, _lib.SOLVER_TIMERS, C.byref(h)) == 0
+    A, d, rhs = _generate(n, seed, n)
+    x = empty(n, torch.float64)
+    tim = np.zeros(5)
+    assert L.omx_lu_factor(h, ptr(A), n, None, tim.ctypes.data) == 0
+    out = np.zeros(4)
+    trace = np.zeros(100)
+    assert L.omx_ir_solve(h, ptr(A), n, seed, ptr(d), ptr(rhs), ptr(x), method, 50, 50, 1e-10, 16.0, 0,
+                          out.ctypes.data, trace.ctypes.data) == 0
+    sec = np.zeros(_lib.TIMER_ITEMS)
+    ops = np.zeros(_lib.TIMER_ITEMS)
+    assert L.omx_solver_timers(h, sec.ctypes.data, ops.ctypes.data) == 0
+    L.omx_solver_destroy(h)
+    name = {L.omx_timer_name(i).decode(): i for i in range(_lib.TIMER_ITEMS)}
+    for item in ("DIAG_LU", "TRSM_L", "TRSM_R", "CONV_L", "CONV_R", "GEMM_UPDATE", "IR_GEMV", "IR_TRSV", "TOTAL"):
+        assert sec[name[item]] > 0.0, item
+    for item in ("DIAG_BCAST", "LCOL_BCAST", "RROW_BCAST", "IR_GEMV_COMM", "IR_TRSV_COMM", "INIT"):
+        assert sec[name[item]] == 0.0 and ops[name[item]] == 0.0, item
+    nb = n // b
+    schur = sum(2.0 * b * (n - (k + 1) * b) ** 2 for k in range(nb))
+    assert ops[name["GEMM_UPDATE"]] == pytest.approx(schur, rel=1e-12)
+    inverses = (nb - 2) if alt else 0  # every panel but the first and the last
+    assert ops[name["DIAG_LU"]] == pytest.approx(2.0 / 3.0 * b ** 3 * (nb + inverses), rel=1e-12)
+    # out[2] corrections, out[3] inner GMRES steps (0 for classical IR): a
+    # residual per check and per Arnoldi step, a pair of solves per
+    # correction (GMRES: its first basis vector) and per Arnoldi step
+    gemvs = out[2] + 1 + out[3]
+    pairs = out[2] + out[3]
+    assert ops[name["IR_GEMV"]] == pytest.approx(2.0 * n * n * gemvs, rel=1e-12)
+    assert ops[name["IR_TRSV"]] == pytest.approx(2.0 * n * n * pairs, rel=1e-12)
+    # MISC closes the sum to TOTAL
+    assert sec[name["MISC"]] == pytest.approx(sec[name["TOTAL"]] - sec[:name["TOTAL"]].sum(), abs=1e-12)
+
+
+def test_phase_timers_off_by_default():
+    L = _L()
+    h = C.c_void_p()
+    assert L.omx_solver_create(1024, 256, 1, 0, C.byref(h)) == 0
+    buf = np.zeros(32)
+    assert L.omx_solver_timers(h, buf.ctypes.data, buf.ctypes.data) < 0
+    L.omx_solver_destroy(h)
+
