@@ -143,6 +143,41 @@ int omx_randsvd_f32(int64_t n, int64_t rank, const double *sigma,
                     double *diag, double *rowabs, void *work,
                     size_t work_bytes, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * The Fasi-Higham HPL-AI matrix, the reference's --highammat family
+ * (main.cpp:239-240, highammgen.hpp:1-31, higham_mat_impl.cpp:37-160, after
+ * github.com/higham/hpl-ai-matrix): A = L U with L unit lower, l_ij = -alpha
+ * below the diagonal, and U unit upper, u_ij = -beta above it, so
+ *   a_ij = -beta + alpha beta i (i < j),  1 + alpha beta i (i = j),
+ *          -alpha + alpha beta j (i > j)          (0-based i, j),
+ * computed in FP64 as written (product, then sum).  The no-pivot LU is
+ * exact; alpha = rho beta with beta set so that the paper's estimate of
+ * cond_inf(A) equals kappa (higham_mat_impl.cpp:143-148).  alpha and beta
+ * are signed: the reference's panel generator (matgen.hpp:157-208, with
+ * hmg_gemv_*, otf_gemv.cpp:135-316, and higham_infnorm,
+ * panel_norm.hpp:137-192) builds L = I + alpha N, U = I + beta N, which is
+ * this matrix at (-alpha, -beta).
+ * ------------------------------------------------------------------------- */
+/* beta for order n, target kappa and rho = alpha / beta (bisection on the
+ * paper's cond_inf estimate; higham_mat_comp_beta, higham_mat_impl.cpp:37-128,
+ * with its integer products in 64 bits); NaN if no beta attains kappa. */
+double omx_higham_beta(int64_t n, double kappa, double rho);
+/* The paper's cond_inf(A) estimate for (alpha, beta) (fhpl,
+ * higham_mat_impl.cpp:37-66). */
+double omx_higham_cond_est(int64_t n, double alpha, double beta);
+/* ||A||_inf of the FP64 matrix (host, O(n); higham_infnorm,
+ * panel_norm.hpp:137-192). */
+double omx_higham_norm_inf(int64_t n, double alpha, double beta);
+/* A (device, FP32, column-major, lda >= n) = the matrix rounded to FP32;
+ * diag (device, n) = its FP64 diagonal (hplai_matrix_impl,
+ * higham_mat_impl.cpp:130-141). */
+int omx_higham_matgen_f32(int64_t n, double alpha, double beta, float *A,
+                          int64_t lda, double *diag, void *stream);
+/* y += s A x with the FP64 matrix regenerated on the fly (hmg_gemv_up /
+ * hmg_gemv_low, otf_gemv.cpp:135-316); x, y device. */
+int omx_higham_gemv_f64(int64_t n, double alpha, double beta, double s,
+                        const double *x, double *y, void *stream);
+
 /* Bytes of scratch omx_lowrank_gemv_f64 needs. */
 size_t omx_lowrank_gemv_workspace_bytes(int64_t rank);
 
@@ -514,10 +549,13 @@ int omx_ir_solve(omx_solver *s, const float *LU, int64_t lda, uint64_t seed,
 /* The system a refinement solves.  kind OMX_SYSTEM_LCG: the reference
  * generator (seed; ||A||_inf = 2 max diag, panel_norm.hpp:129-135).
  * kind OMX_SYSTEM_LOWRANK: diag(sigma) + P Q^T (rank, sigma, P, Q device;
- * norm_inf = ||A||_inf given by the caller).  diag (device, n) is A's FP64
- * diagonal for both kinds (x0 = rhs ./ diag). */
+ * norm_inf = ||A||_inf given by the caller).  kind OMX_SYSTEM_HIGHAM: the
+ * Fasi-Higham matrix (alpha, beta; norm_inf given by the caller, e.g.
+ * omx_higham_norm_inf).  diag (device, n) is A's FP64 diagonal for every
+ * kind (x0 = rhs ./ diag). */
 #define OMX_SYSTEM_LCG 0
 #define OMX_SYSTEM_LOWRANK 1
+#define OMX_SYSTEM_HIGHAM 2
 typedef struct omx_system {
     int kind;
     uint64_t seed;
@@ -527,6 +565,7 @@ typedef struct omx_system {
     const double *sigma;
     const double *P;
     const double *Q;
+    double alpha, beta; /* OMX_SYSTEM_HIGHAM */
 } omx_system;
 
 /* omx_ir_solve for any omx_system (same arguments and results). */

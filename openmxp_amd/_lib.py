@@ -37,6 +37,11 @@ SIGNATURES = {
     "omx_randsvd_factors": (i32, [i64, u64, dbl, i32, vp, vp, vp, vp]),
     "omx_randsvd_workspace_bytes": (sz, [i64]),
     "omx_randsvd_f32": (i32, [i64, i64, vp, vp, vp, vp, i64, vp, vp, vp, sz, vp]),
+    "omx_higham_beta": (dbl, [i64, dbl, dbl]),
+    "omx_higham_cond_est": (dbl, [i64, dbl, dbl]),
+    "omx_higham_norm_inf": (dbl, [i64, dbl, dbl]),
+    "omx_higham_matgen_f32": (i32, [i64, dbl, dbl, vp, i64, vp, vp]),
+    "omx_higham_gemv_f64": (i32, [i64, dbl, dbl, dbl, vp, vp, vp]),
     "omx_lowrank_gemv_workspace_bytes": (sz, [i64]),
     "omx_lowrank_gemv_f64": (i32, [i64, i64, vp, vp, vp, dbl, vp, vp, vp, sz, vp]),
     "omx_trsv_workspace_bytes": (sz, [i64]),
@@ -88,7 +93,8 @@ SIGNATURES = {
 class System(C.Structure):
     """struct omx_system (include/omxp.h)."""
     _fields_ = [("kind", C.c_int), ("seed", C.c_uint64), ("diag", vp), ("norm_inf", C.c_double),
-                ("rank", C.c_int64), ("sigma", vp), ("P", vp), ("Q", vp)]
+                ("rank", C.c_int64), ("sigma", vp), ("P", vp), ("Q", vp), ("alpha", C.c_double),
+                ("beta", C.c_double)]
 
 
 SOLVER_FP8 = 1      # omx_solver_create flags
@@ -99,6 +105,7 @@ TIMER_ITEMS = 23
 
 SYSTEM_LCG = 0
 SYSTEM_LOWRANK = 1
+SYSTEM_HIGHAM = 2
 
 _lib = None
 

@@ -39,6 +39,7 @@ LIB_SOURCES = [
     "omx_dense.hip",
     "omx_randsvd.hip",
     "omx_mlp.hip",
+    "omx_higham.hip",
     "omx_solver.cpp",
 ]
 DRIVER_SOURCES = ["omxp_main.cpp"]

@@ -701,6 +701,7 @@ static int system_gemv(omx_solver *s, const omx_system *sys, double alpha, const
                        hipStream_t st) {
     if (sys->kind == OMX_SYSTEM_LCG)
         return omx_otf_gemv_f64(s->n, sys->seed, sys->diag, alpha, x, y, s->gemv_ws.p, s->gemv_ws.bytes, st);
+    if (sys->kind == OMX_SYSTEM_HIGHAM) return omx_higham_gemv_f64(s->n, sys->alpha, sys->beta, alpha, x, y, st);
     return omx_lowrank_gemv_f64(s->n, sys->rank, sys->sigma, sys->P, sys->Q, alpha, x, y, s->gemv_ws.p,
                                 s->gemv_ws.bytes, st);
 }
@@ -719,10 +720,11 @@ static int ir_solve_impl(omx_solver *s, const float *LU, int64_t lda, const omx_
                          double *x, int method, int maxit, int restart, double inner_tol, double stop_harness,
                          int x_given, double *out, double *trace) {
     if (!s || !LU || !sys || !sys->diag || !rhs || !x || maxit <= 0 || lda < s->n) return OMX_ERR_ARG;
-    if (sys->kind != OMX_SYSTEM_LCG &&
-        (sys->kind != OMX_SYSTEM_LOWRANK || sys->rank < 0 || sys->rank > 16 || !sys->sigma ||
-         (sys->rank > 0 && (!sys->P || !sys->Q)) || !(sys->norm_inf > 0)))
-        return OMX_ERR_ARG;
+    const bool lowrank_ok = sys->kind == OMX_SYSTEM_LOWRANK && sys->rank >= 0 && sys->rank <= 16 && sys->sigma &&
+                            (sys->rank == 0 || (sys->P && sys->Q)) && sys->norm_inf > 0;
+    const bool higham_ok = sys->kind == OMX_SYSTEM_HIGHAM && std::isfinite(sys->alpha) &&
+                           std::isfinite(sys->beta) && sys->norm_inf > 0;
+    if (sys->kind != OMX_SYSTEM_LCG && !lowrank_ok && !higham_ok) return OMX_ERR_ARG;
     const double *diag = sys->diag;
     if (method == 1 && (restart <= 0 || restart > 512)) return OMX_ERR_ARG;
     const int64_t n = s->n, b = s->b;

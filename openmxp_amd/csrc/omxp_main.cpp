@@ -1,7 +1,8 @@
 // omxp_main.cpp -- command-line driver with the reference's surface
 // (main.cpp:85-252, :273-736): positional N B [P Q] and the flags -solv,
 // -alt, -log, -progress, -init, plus -prec (fp16 | fp8 | learned), -policy,
-// -ir, -seed, -maxit, -restart, -device.  One GPU, 1x1 grid.  With
+// -ir, -seed, -maxit, -restart, -device, and --highammat (the Fasi-Higham
+// matrix family, main.cpp:239-240, highammgen.hpp:4-30).  One GPU, 1x1 grid.  With
 // -prec learned the MLP policy picks each step's Schur precision; if the
 // refinement then fails, the system is generated again and factored in FP16
 // (inside the timed region).  The timed region is LU + IR, generation
@@ -30,6 +31,12 @@ struct Opts {
     int ir = 0, maxit = 50, restart = 50, device = 0;
     int group = 0;  // panels per trailing update: 0 = 4 for fp8/learned, 2 for fp16
     int timer = 0;  // print the phase timers (Timer::show, timer.hpp:160-182) after the run
+    // --highammat: 1 = the reference's panel generator, L = I + alpha N and
+    // U = I + beta N with N all ones strictly below / above the diagonal
+    // (matgen.hpp:157-208); 2 = the paper's form with -alpha, -beta, whose
+    // cond_inf is kappa (higham_mat_impl.cpp:130-149)
+    int higham = 0;
+    double cond = 1e3, rho = 0.5;  // kappa and alpha / beta (higham_mat_impl.cpp:143-148)
     double inner_tol = 1e-10;
     unsigned long long seed = 42;
     std::string prec = "fp16";
@@ -40,7 +47,8 @@ void usage() {
     std::printf(
         "usage: omxp N B [P Q] [-solv 0|1] [-alt 0|1|2] [-log 0|1] [-progress k] [-init 1]\n"
         "            [-prec fp16|fp8|learned] [-policy file] [-ir classic|gmres] [-seed s] [-maxit k]\n"
-        "            [-restart m] [-device d] [-group 2|4] [-timer]\n");
+        "            [-restart m] [-device d] [-group 2|4] [-timer]\n"
+        "            [--highammat | --highammat-paper] [-cond kappa] [-rho r]\n");
 }
 
 bool parse(int argc, char **argv, Opts &o) {
@@ -76,6 +84,10 @@ bool parse(int argc, char **argv, Opts &o) {
         else if (!std::strcmp(a, "-device")) o.device = std::atoi(next(a));
         else if (!std::strcmp(a, "-group")) o.group = std::atoi(next(a));
         else if (!std::strcmp(a, "-timer")) o.timer = 1;
+        else if (!std::strcmp(a, "--highammat")) o.higham = 1;
+        else if (!std::strcmp(a, "--highammat-paper")) o.higham = 2;
+        else if (!std::strcmp(a, "-cond")) o.cond = std::atof(next(a));
+        else if (!std::strcmp(a, "-rho")) o.rho = std::atof(next(a));
         else if (!std::strcmp(a, "-h") || !std::strcmp(a, "--help")) {
             usage();
             std::exit(0);
@@ -133,6 +145,19 @@ int main(int argc, char **argv) {
     }
     const bool learned = (o.prec == "learned");
     const bool fp8 = (o.prec == "fp8") || learned;
+    // the Higham matrix in the library's signed form: a_ij = -beta + ab i
+    // above, 1 + ab i on, -alpha + ab j below the diagonal
+    double h_alpha = 0, h_beta = 0;
+    if (o.higham) {
+        const double beta = omx_higham_beta(o.n, o.cond, o.rho);
+        if (!std::isfinite(beta) || !(beta > 0)) {
+            std::fprintf(stderr, "no Higham beta for n=%ld kappa=%g rho=%g\n", o.n, o.cond, o.rho);
+            return 2;
+        }
+        const double sign = o.higham == 1 ? -1.0 : 1.0;
+        h_alpha = sign * o.rho * beta;
+        h_beta = sign * beta;
+    }
     std::vector<float> params;
     if (learned) {
         if (o.policy.empty()) {
@@ -158,6 +183,10 @@ int main(int argc, char **argv) {
     std::printf("\tn=%ld b=%ld 1x1 grid, alt=%d, Schur precision %s, IR %s (maxit %d)\n", n, b, o.alt,
                 learned ? "learned FP8/FP16 per step" : (fp8 ? "FP8 e4m3 (pow2 tile scale)" : "FP16"),
                 o.ir ? "GMRES" : "classical", o.maxit);
+    if (o.higham)
+        std::printf("\tFasi-Higham matrix (%s): kappa=%g rho=%g alpha=%.6e beta=%.6e\n",
+                    o.higham == 1 ? "L = I + alpha N, U = I + beta N" : "L = I - alpha N, U = I - beta N", o.cond,
+                    o.rho, std::fabs(h_alpha), std::fabs(h_beta));
 
     float *A = nullptr;
     double *d = nullptr, *rhs = nullptr, *x = nullptr;
@@ -173,8 +202,19 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "solver creation failed\n");
         return 1;
     }
+    auto generate = [&]() {
+        return o.higham ? omx_higham_matgen_f32(n, h_alpha, h_beta, A, lda, d, nullptr)
+                        : omx_matgen_f32(n, o.seed, A, lda, d, nullptr);
+    };
+    omx_system sys{};
+    sys.kind = o.higham ? OMX_SYSTEM_HIGHAM : OMX_SYSTEM_LCG;
+    sys.seed = o.seed;
+    sys.diag = d;
+    sys.alpha = h_alpha;
+    sys.beta = h_beta;
+    sys.norm_inf = o.higham ? omx_higham_norm_inf(n, h_alpha, h_beta) : 0.0;
     double t = now_s();
-    if (omx_matgen_f32(n, o.seed, A, lda, d, nullptr) || omx_rhs_f64(n, o.seed, rhs, nullptr)) {
+    if (generate() || omx_rhs_f64(n, o.seed, rhs, nullptr)) {
         std::fprintf(stderr, "generation failed\n");
         return 1;
     }
@@ -204,17 +244,17 @@ int main(int argc, char **argv) {
     const double start_ir = now_s();
     double out[4] = {0};
     std::vector<double> trace(2 * o.maxit, 0.0);
-    int rc = omx_ir_solve(s, A, lda, o.seed, d, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, 16.0, 0, out,
-                          trace.data());
+    int rc = omx_ir_solve_system(s, A, lda, &sys, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, 16.0, 0, out,
+                                 trace.data());
     if (learned && rc != 0) {
         // fallback: the learned schedule's factors did not refine; FP16
         std::printf("learned schedule did not converge: regenerating and factoring in FP16\n");
-        if (omx_matgen_f32(n, o.seed, A, lda, d, nullptr)) return 1;
+        if (generate()) return 1;
         CK(hipDeviceSynchronize());
         std::fill(sched.begin(), sched.end(), 0);
         info = omx_lu_factor(s, A, lda, sched.data(), tim);
-        rc = omx_ir_solve(s, A, lda, o.seed, d, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, 16.0, 0, out,
-                          trace.data());
+        rc = omx_ir_solve_system(s, A, lda, &sys, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, 16.0, 0, out,
+                                 trace.data());
     }
     const double stop = now_s();
     const int steps = (rc == 0) ? (int)out[2] + 1 : o.maxit;

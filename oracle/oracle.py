@@ -112,6 +112,9 @@ def ref_lib():
             "ref_shgemm": (None, [C.c_int, C.c_int, C.c_int, flt, vp, C.c_int, vp, C.c_int, flt, vp, C.c_int]),
             "ref_sgetrf_nopiv_tuned": (None, [C.c_int, vp, C.c_size_t]),
             "ref_sgetrf_nopiv_naive": (None, [C.c_int, C.c_int, vp, C.c_size_t]),
+            "ref_higham_fhpl": (dbl, [C.c_int, dbl, dbl]),
+            "higham_mat_comp_beta": (dbl, [C.c_int, dbl, dbl]),
+            "hplai_matrix_impl": (None, [C.c_int, vp, C.c_int, dbl, dbl]),
         }
         for name, (res, args) in sig.items():
             f = getattr(R, name)

@@ -18,6 +18,11 @@ int grank = 0;
 // compiled from the reference file into the same .so
 void sgetrf_nopiv_tuned(int n, float *a, size_t lda);
 
+// higham_mat_impl.cpp:36-64, compiled from the reference file into the
+// same .so (the extern "C" routines of that file, higham_mat_comp_beta and
+// hplai_matrix_impl, are exported as they are)
+double fhpl(int n, double alpha, double beta);
+
 extern "C" {
 
 // RandStat of linear index i + n*j, as double (hpl_rand.hpp:54-65, :249-254)
@@ -88,5 +93,8 @@ void ref_shgemm(int m, int n, int k, float alpha, const uint16_t *a, int lda,
 void ref_sgetrf_nopiv_tuned(int n, float *a, size_t lda) {
     sgetrf_nopiv_tuned(n, a, lda);
 }
+
+// fhpl (higham_mat_impl.cpp:36-64): the cond_inf estimate of A(n, a, b)
+double ref_higham_fhpl(int n, double alpha, double beta) { return fhpl(n, alpha, beta); }
 
 }  // extern "C"
