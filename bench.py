@@ -212,13 +212,16 @@ def cpu_baseline(n, b, alt, seed):
                 iters=res["iters"], info=info)
 
 
-def pmc_traffic(tag):
+def pmc_traffic(tag, n, b, group):
     """HBM bytes per launch of the FP16 ("f16") or FP8 ("f8") trailing-update
     GEMM from the committed PMC summary (rocprofv3 FETCH_SIZE / WRITE_SIZE,
-    copy-calibrated), if any."""
+    copy-calibrated), if any, and only for the workload it was measured on
+    (its n and b; FP16 in pairs, FP8 in groups of four)."""
     try:
         txt = open(PMC_SUMMARY).read()
     except OSError:
+        return None
+    if not re.search(rf"\bn={n} b={b}\b", txt) or group != {"f16": 2, "f8": 4}[tag]:
         return None
     m = re.search(tag + r"\.traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
     return float(m.group(1)) if m else None
@@ -400,7 +403,7 @@ def main():
         cpu = cpu_baseline(args.cpu_n, args.cpu_b, args.alt, args.seed)
     head = aL if aL else a16
     roof16 = {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
-              "frac": round(achieved / PEAK_FP16_TFLOPS, 4), "traffic": pmc_traffic("f16"),
+              "frac": round(achieved / PEAK_FP16_TFLOPS, 4), "traffic": pmc_traffic("f16", n, b, args.group_plain),
               "kernel": f"k_gemm_f16 trailing update per {args.group_plain} panels (depth {args.group_plain}b, "
                         "alpha=-1, beta=1), plain run",
               "launches": a16["gemm_launches"]}
@@ -440,7 +443,7 @@ def main():
         # the headline run's dominant kernel: its trailing updates (FP8 where
         # the schedule chose FP8; all of them when fp8_steps == steps)
         line["roofline"] = {"bound": "mfma", "achieved": round(achL, 1), "peak": PEAK_FP8_TFLOPS, "unit": "TFLOP/s",
-                            "frac": round(achL / PEAK_FP8_TFLOPS, 4), "traffic": pmc_traffic("f8"),
+                            "frac": round(achL / PEAK_FP8_TFLOPS, 4), "traffic": pmc_traffic("f8", n, b, args.group),
                             "kernel": f"k_gemm_f8 trailing update per {args.group} panels (depth {args.group}b, "
                                       "alpha=-1, beta=1), learned run",
                             "launches": aL["gemm_launches"],
