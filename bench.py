@@ -12,12 +12,17 @@ synchronisations.  Throughput is (2/3 n^3 + 3/2 n^2) / time
 overwrites the matrix; a rate is steps * flops / sum of step times.
 
 Two methods are measured on the same system:
-  plain    FP16 Schur update, classical IR (the reference's method);
+  plain    FP16 Schur update, classical IR (the reference's method; the
+           trailing matrix updated per pair of panels);
   learned  the MLP policy (openmxp_amd/data/sched_mlp.f32) picks FP8 or FP16
-           per LU step, GMRES-IR; if the refinement fails the system is
-           generated again and factored in FP16, inside the timed region.
+           per LU step, GMRES-IR, trailing update per group of four panels;
+           if the refinement fails the system is generated again and
+           factored in FP16, inside the timed region.
 `value` is the learned method's rate; the line carries the plain rate and
-the ratio.  `roofline` is that of the headline run's dominant kernel, the
+the ratio.  Two attribution legs separate the ratio's parts (--attr-steps
+timed solves each): plain FP16 with groups of four (the grouping gain) and
+FP8 on every step with groups of four and GMRES-IR (the FP8 gain; the
+learned leg over it is the policy's own gain).  `roofline` is that of the headline run's dominant kernel, the
 FP8 trailing-update GEMM of the learned run (`roofline_plain_fp16`: the FP16
 one of the plain run), each timed with HIP events on the stream its launches
 run on, with the HBM bytes per launch from the committed rocprofv3 counter
@@ -61,7 +66,9 @@ def parse():
                     help="panels per trailing update for the learned (FP8) solver")
     ap.add_argument("--group-plain", type=int, choices=(2, 4), default=2,
                     help="panels per trailing update for the plain FP16 solver")
-    ap.add_argument("--fp8-all", action="store_true", help="also measure FP8 on every step")
+    ap.add_argument("--attr-steps", type=int, default=2,
+                    help="timed solves of each attribution leg (plain FP16 in groups of four; FP8 on every step)")
+    ap.add_argument("--no-attr", action="store_true", help="skip the attribution legs")
     ap.add_argument("--cpu-n", type=int, default=10240, help="order of the CPU baseline sample")
     ap.add_argument("--cpu-b", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
@@ -199,6 +206,16 @@ def agg(res, flops):
                 steps_total=last["steps_total"], fell_back=any(r["fell_back"] for r in res))
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n, b, alt, seed):
     import oracle
     threads = oracle.get_threads()
@@ -208,8 +225,26 @@ def cpu_baseline(n, b, alt, seed):
     LU, info = oracle.lu_factor(A, b, alt=alt)
     res = oracle.ir_classic(LU, b, seed, d, rhs)
     t = time.perf_counter() - t0
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
     return dict(value=hpl_flops(n) / t * 1e-12, seconds=t, threads=threads, harness=res["harness"],
-                iters=res["iters"], info=info)
+                iters=res["iters"], info=info, model=cpu_model(), nproc=affinity,
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
+
+
+def gpu_same_sample(n, b, alt, seed):
+    """The plain GPU method on the CPU baseline's sample (n, b): generate,
+    then LU + classical IR timed as the bench's steps are."""
+    gs = GpuSolve(n, b, alt, fp8=False, group=2)
+    rates = []
+    for _ in range(3):
+        gs.generate(seed)
+        r = gs.solve(seed)
+        rates.append(hpl_flops(n) / r["seconds"] * 1e-12)
+    gs.close()
+    return dict(tflops=max(rates), harness=r["harness"], ir_steps=r["ir_steps"])
 
 
 def pmc_traffic(tag, n, b, group):
@@ -386,21 +421,31 @@ def main():
     a16 = agg(res16, flops)
     aL = a8 = None
     fwdL = fwd8 = None
-    if policy is not None or args.fp8_all:
+    attr = argparse.Namespace(**vars(args))
+    attr.steps, attr.warmup = max(1, args.attr_steps), min(args.warmup, 1)
+    a16g = fwd16g = None
+    if not args.no_attr and args.group_plain != 4:
+        gs = GpuSolve(n, b, args.alt, fp8=False, group=4)
+        res16g, fwd16g = measure(gs, attr, "plain fp16 groups of 4 + IR", None, 0)
+        gs.close()
+        del gs
+        a16g = agg(res16g, flops)
+    if policy is not None or not args.no_attr:
         gs = GpuSolve(n, b, args.alt, fp8=True, policy=policy, group=args.group)
         if policy is not None:
             resL, fwdL = measure(gs, args, "learned + GMRES-IR", None, 1, True)
             aL = agg(resL, flops)
-        if args.fp8_all:
-            res8, fwd8 = measure(gs, args, "fp8 + GMRES-IR", [1] * (n // b), 1)
+        if not args.no_attr:
+            res8, fwd8 = measure(gs, attr, "fp8 every step + GMRES-IR", [1] * (n // b), 1)
             a8 = agg(res8, flops)
         gs.close()
         del gs
     achieved = a16["gemm_flops"] / (a16["gemm_ms"] * 1e-3) * 1e-12
-    cpu = None
+    cpu = gpu_small = None
     if not args.no_cpu:
         log(f"CPU baseline: oracle solve at n={args.cpu_n} b={args.cpu_b}")
         cpu = cpu_baseline(args.cpu_n, args.cpu_b, args.alt, args.seed)
+        gpu_small = gpu_same_sample(args.cpu_n, args.cpu_b, args.alt, args.seed)
     head = aL if aL else a16
     roof16 = {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
               "frac": round(achieved / PEAK_FP16_TFLOPS, 4), "traffic": pmc_traffic("f16", n, b, args.group_plain),
@@ -450,19 +495,46 @@ def main():
                             "fp8_steps": aL["fp8_steps"], "steps": aL["steps_total"]}
         line["learned"] = {"speedup_vs_plain_fp16": round(aL["tflops"] / a16["tflops"], 3),
                            "fp8_steps": aL["fp8_steps"], "steps": aL["steps_total"], "inner_iters": aL["inner"],
+                           "policy_equals_fp8_all": aL["fp8_steps"] == aL["steps_total"],
                            "fell_back_to_fp16": aL["fell_back"], "trailing_gemm_tflops": round(achL, 1),
                            "trailing_gemm_frac_fp8_peak": round(achL / PEAK_FP8_TFLOPS, 4)}
+        if a16g:
+            line["learned"]["speedup_vs_plain_fp16_g4"] = round(aL["tflops"] / a16g["tflops"], 3)
+        if a8:
+            line["learned"]["speedup_vs_fp8_all"] = round(aL["tflops"] / a8["tflops"], 3)
+    if a16g:
+        achg = a16g["gemm_flops"] / (a16g["gemm_ms"] * 1e-3) * 1e-12
+        line["plain_fp16_g4"] = {"tflops": round(a16g["tflops"], 3), "ms_per_step": round(a16g["ms"], 2),
+                                 "steps": attr.steps, "valid": a16g["ok"], "harness": a16g["harness"],
+                                 "ir_steps": a16g["ir_steps"], "forward_error": fwd16g,
+                                 "lu_s": round(a16g["lu_s"], 4), "ir_s": round(a16g["ir_s"], 4),
+                                 "trailing_gemm_tflops": round(achg, 1),
+                                 "trailing_gemm_frac_fp16_peak": round(achg / PEAK_FP16_TFLOPS, 4)}
     if a8:
         ach8 = a8["gemm_flops"] / (a8["gemm_ms"] * 1e-3) * 1e-12
         line["fp8_all"] = {"tflops": round(a8["tflops"], 3), "ms_per_step": round(a8["ms"], 2), "valid": a8["ok"],
-                           "harness": a8["harness"], "ir_steps": a8["ir_steps"], "forward_error": fwd8,
+                           "steps": attr.steps, "harness": a8["harness"], "ir_steps": a8["ir_steps"],
+                           "inner_iters": a8["inner"], "forward_error": fwd8,
                            "speedup_vs_plain_fp16": round(a8["tflops"] / a16["tflops"], 3),
                            "gemm_tflops": round(ach8, 1), "gemm_frac_fp8_peak": round(ach8 / PEAK_FP8_TFLOPS, 4)}
+    if aL and a16g and a8:
+        # value / plain = (G4 / G2 plain) x (all-FP8 / plain G4) x (learned / all-FP8)
+        line["attribution"] = {"grouping_gain": round(a16g["tflops"] / a16["tflops"], 3),
+                               "fp8_gain": round(a8["tflops"] / a16g["tflops"], 3),
+                               "policy_gain": round(aL["tflops"] / a8["tflops"], 3),
+                               "note": "value / plain_fp16 = grouping_gain x fp8_gain x policy_gain; the two middle "
+                                       "legs are timed over attr steps"}
     if cpu:
         line["cpu_baseline"] = {"value": round(cpu["value"], 5), "unit": "TFLOP/s", "cores": cpu["threads"],
                                 "kind": "port",
                                 "sample": f"oracle (C/OpenMP) full LU+IR at n={args.cpu_n} b={args.cpu_b}, "
-                                          f"{cpu['seconds']:.1f} s, harness {cpu['harness']:.3f}"}
+                                          f"{cpu['seconds']:.1f} s, harness {cpu['harness']:.3f}",
+                                "cpu_model": cpu["model"], "nproc": cpu["nproc"],
+                                "omp_num_threads": cpu["omp_num_threads"],
+                                "gpu_same_sample_tflops": round(gpu_small["tflops"], 3),
+                                "projected_cpu_seconds_at_n": {
+                                    "n": n, "seconds": round(hpl_flops(n) / (cpu["value"] * 1e12), 1),
+                                    "note": "projected: the measured CPU rate applied to the headline n, not run"}}
     print(json.dumps(line), flush=True)
 
 

@@ -462,9 +462,9 @@ int omx_sched_features_f32(int64_t ntiles_l, const float *lmax,
 /* ---------------------------------------------------------------------------
  * Solver: the blocked LU with look-ahead and the FP64 refinement, driven by
  * the host.  A handle owns two HIP streams, events and the panel and vector
- * workspaces (allocated once by omx_solver_create; the factorisation and
- * refinement calls allocate nothing except GMRES's basis on first use of a
- * larger restart).  Calls on one handle must not overlap.
+ * workspaces, GMRES's basis included (allocated once by omx_solver_create,
+ * or enlarged by omx_solver_reserve_gmres; the factorisation and refinement
+ * calls allocate nothing).  Calls on one handle must not overlap.
  * ------------------------------------------------------------------------- */
 typedef struct omx_solver omx_solver;
 
@@ -481,6 +481,16 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags,
                       omx_solver **out);
 int omx_solver_destroy(omx_solver *s);
 
+/* GMRES-IR's basis: omx_solver_create sizes it for restarts up to
+ * OMX_GMRES_RESTART_DEFAULT ((restart + 1) n doubles: 82 MB at n = 204800).
+ * A larger restart (up to OMX_GMRES_RESTART_MAX) needs this call first; it
+ * allocates (set-up, not a hot call) and omx_ir_solve then returns
+ * OMX_ERR_WORKSPACE for restarts above the reserved one.  The reference has
+ * no GMRES (its refinement is classical, iterative_refinement.hpp:141-171). */
+#define OMX_GMRES_RESTART_DEFAULT 50
+#define OMX_GMRES_RESTART_MAX 512
+int omx_solver_reserve_gmres(omx_solver *s, int restart);
+
 /* In-place no-pivot LU of the order-n matrix A (device, lda >= n,
  * lda % 4 == 0) in the step order of panel_lu_async_wGPU_gdirect_alt2
  * (panel_trf_gdirect.hpp:593-1192) on one GPU, with the diagonal/panel work
@@ -495,6 +505,26 @@ int omx_solver_destroy(omx_solver *s);
  * first zero pivot. */
 int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched,
                   double *timings);
+
+/* The reference's -skip bit-mask (main.cpp:68-70, applied in
+ * panel_trf_gdirect.hpp:707,833,901 and panel_trf.hpp:198,237): bit 0 skips
+ * the refinement (the driver's business: the library ignores it), bit 1 the
+ * diagonal getrf, bit 2 the panel solves (TRSM or -alt inverse products),
+ * bit 3 the Schur GEMMs.  The factors are then wrong: a fault-injection and
+ * timing aid, as in the reference.  mask 0 (the default) does everything. */
+#define OMX_SKIP_IR 1
+#define OMX_SKIP_GETRF 2
+#define OMX_SKIP_TRSM 4
+#define OMX_SKIP_GEMM 8
+int omx_solver_set_skip(omx_solver *s, int mask);
+
+/* After omx_lu_factor: ms[k] (host, n/b doubles) = milliseconds from the
+ * start of the factorisation until every Schur product of step k's panels
+ * was done (the steps of one group share a time).  The reference prints a
+ * projected-rate line from the same quantity every -progress steps, after a
+ * device synchronisation per step (panel_trf_gdirect.hpp:1146-1161); here
+ * the times come from events and the driver prints the lines afterwards. */
+int omx_solver_step_times(omx_solver *s, double *ms);
 
 /* Set (params: host array of OMX_MLP_NPARAMS floats) or clear (NULL) the
  * learned schedule used by omx_lu_factor when sched is NULL.  Needs a solver
@@ -539,7 +569,9 @@ const char *omx_timer_name(int item);
  * Blocks until done.  out (host, 4 doubles): final residual,
  * harness, corrections applied (-1 if not converged), total inner
  * iterations.  trace (host, 2*maxit, optional): residual and harness per
- * step.  Returns 0, or OMX_ERR_NOCONV. */
+ * step.  Returns 0, OMX_ERR_NOCONV, OMX_ERR_ARG for a method other than 0
+ * or 1 (checked before any work is queued), or OMX_ERR_WORKSPACE for a
+ * GMRES restart above the reserved capacity. */
 int omx_ir_solve(omx_solver *s, const float *LU, int64_t lda, uint64_t seed,
                  const double *diag, const double *rhs, double *x,
                  int method, int maxit, int restart, double inner_tol,

@@ -2,8 +2,9 @@
 
 The product is the C ABI in include/omxp.h, implemented by hand-written HIP
 kernels and a C++ host driver in openmxp_amd/csrc (libomxp.so, bin/omxp).
-This Python package only loads that library (``openmxp_amd._lib``) and wraps
-it for tests and the benchmark (``openmxp_amd.api``).
+This Python package loads that library through ctypes (``openmxp_amd._lib``,
+used by the tests and the benchmark), builds it (``openmxp_amd.build``) and
+trains the learned precision schedule (``openmxp_amd.scheduler``).
 """
 from ._lib import OmxError, check, lib  # noqa: F401
 

@@ -78,6 +78,9 @@ SIGNATURES = {
     "omx_sched_features_f32": (i32, [i64, vp, i64, vp, vp, i64, i64, i64, i64, i64, i32, vp, vp]),
     "omx_solver_create": (i32, [i64, i64, i32, i32, C.POINTER(vp)]),
     "omx_solver_destroy": (i32, [vp]),
+    "omx_solver_reserve_gmres": (i32, [vp, i32]),
+    "omx_solver_set_skip": (i32, [vp, i32]),
+    "omx_solver_step_times": (i32, [vp, vp]),
     "omx_lu_factor": (i32, [vp, vp, i64, vp, vp]),
     "omx_solver_set_policy": (i32, [vp, vp]),
     "omx_solver_last_schedule": (i32, [vp, vp, vp]),
@@ -130,9 +133,11 @@ def lib():
         except ImportError:
             pass
         L = C.CDLL(LIB_PATH)
+        missing = [name for name in SIGNATURES if not hasattr(L, name)]
+        if missing:
+            raise OmxError(f"{LIB_PATH} does not export {', '.join(missing)}: it is older than "
+                           "include/omxp.h; rebuild it with `python -m openmxp_amd.build`")
         for name, (res, args) in SIGNATURES.items():
-            if not hasattr(L, name):
-                continue
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args

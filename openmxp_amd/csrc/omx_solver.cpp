@@ -179,6 +179,14 @@ struct omx_solver {
     // refinement
     DevBuf r, w, gemv_ws, trsv_ws, nrm, V, hv, dots;
     int restart_cap = 0;
+    // the reference's -skip bits 1-3 (main.cpp:68-70): leave out the
+    // diagonal getrf, the panel solves / inverse products, the Schur products
+    int skip = 0;
+    // progress: an event at the start of the factorisation and one per group
+    // when all of the group's Schur products are done
+    hipEvent_t ev_start = nullptr;
+    std::vector<hipEvent_t> ev_grp;
+    int64_t ngrp_done = 0;
     // timings of the last factorisation
     double t_lu_ms = 0, bulk_ms = 0, bulk_flops = 0;
     int bulk_count = 0;
@@ -270,6 +278,22 @@ double wall_ms() {
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+// GMRES basis (restart + 1 vectors of n), Hessenberg column and dot-product
+// partials for restarts up to `restart`; allocates only when the capacity
+// grows (omx_solver_create, omx_solver_reserve_gmres).
+int reserve_gmres(omx_solver *s, int restart) {
+    if (restart <= s->restart_cap) return OMX_OK;
+    dfree(s->V);
+    dfree(s->hv);
+    dfree(s->dots);
+    s->restart_cap = 0;
+    RC(dalloc(s->V, (size_t)(restart + 1) * s->n * 8));
+    RC(dalloc(s->hv, (size_t)(restart + 2) * 8));
+    RC(dalloc(s->dots, (size_t)(restart + 2) * kDotBlocks * 8));
+    s->restart_cap = restart;
+    return OMX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -299,6 +323,10 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
     s->ev_panel.resize(nb);
     s->ev_t0.resize(nb);
     s->ev_t1.resize(nb);
+    if (hipEventCreate(&s->ev_start) != hipSuccess) return fail(OMX_ERR_HIP);
+    s->ev_grp.resize(nb / s->G + 2);
+    for (auto &e : s->ev_grp)
+        if (hipEventCreate(&e) != hipSuccess) return fail(OMX_ERR_HIP);
     for (int64_t k = 0; k < nb; ++k) {
         if (hipEventCreateWithFlags(&s->ev_panel[k], hipEventDisableTiming) != hipSuccess) return fail(OMX_ERR_HIP);
         if (hipEventCreate(&s->ev_t0[k]) != hipSuccess) return fail(OMX_ERR_HIP);
@@ -347,15 +375,22 @@ int omx_solver_create(int64_t n, int64_t b, int alt, int flags, omx_solver **out
         (rc = dalloc(s->gemv_ws, std::max(std::max<size_t>(omx_otf_gemv_workspace_bytes(n), 8), omx_lowrank_gemv_workspace_bytes(16)))) ||
         (rc = dalloc(s->trsv_ws, omx_trsv_workspace_bytes(n))) || (rc = dalloc(s->nrm, 64)))
         return fail(rc);
+    if ((rc = reserve_gmres(s, OMX_GMRES_RESTART_DEFAULT))) return fail(rc);
     *out = s;
     return OMX_OK;
+}
+
+int omx_solver_reserve_gmres(omx_solver *s, int restart) {
+    if (!s || restart <= 0 || restart > OMX_GMRES_RESTART_MAX) return OMX_ERR_ARG;
+    return reserve_gmres(s, restart);
 }
 
 int omx_solver_destroy(omx_solver *s) {
     if (!s) return OMX_OK;
     if (s->crit) (void)hipStreamSynchronize(s->crit);
     if (s->bulk) (void)hipStreamSynchronize(s->bulk);
-    for (auto *v : {&s->ev_panel, &s->ev_t0, &s->ev_t1, &s->ev_la})
+    if (s->ev_start) (void)hipEventDestroy(s->ev_start);
+    for (auto *v : {&s->ev_panel, &s->ev_t0, &s->ev_t1, &s->ev_la, &s->ev_grp})
         for (auto e : *v)
             if (e) (void)hipEventDestroy(e);
     DevBuf *bufs[] = {&s->Lp[0], &s->Lp[1], &s->Rp[0], &s->Rp[1], &s->Tp, &s->L8[0], &s->L8[1], &s->R8[0],
@@ -426,6 +461,7 @@ static int apply_panels(omx_solver *s, int64_t i0, int64_t i1, int64_t r0, int64
 
 // Block row/col t (A_tt, A[t+1:, t], A[t, t+1:]) -= panels i0 .. i1.
 static int update_block(omx_solver *s, int64_t t, int64_t i0, int64_t i1, float *A, int64_t lda, hipStream_t st) {
+    if (s->skip & OMX_SKIP_GEMM) return OMX_OK;
     const int64_t n = s->n, b = s->b, tb = t * b;
     const double fl = 2.0 * (double)(i1 - i0 + 1) * b * b * (2.0 * (n - tb) - b);
     long sp;
@@ -513,6 +549,24 @@ static int policy_decision(omx_solver *s, int *choice) {
     return OMX_OK;
 }
 
+int omx_solver_set_skip(omx_solver *s, int mask) {
+    if (!s || mask < 0 || mask > 15) return OMX_ERR_ARG;
+    s->skip = mask & (OMX_SKIP_GETRF | OMX_SKIP_TRSM | OMX_SKIP_GEMM);
+    return OMX_OK;
+}
+
+int omx_solver_step_times(omx_solver *s, double *ms) {
+    if (!s || !ms || s->ev_grp.empty()) return OMX_ERR_ARG;
+    const int64_t G = s->G;
+    for (int64_t k = 0; k < s->nb; ++k) {
+        const int64_t g = std::min(k / G, s->ngrp_done);
+        float t = 0.0f;
+        OMX_HIP_CHECK(hipEventElapsedTime(&t, s->ev_start, s->ev_grp[g]));
+        ms[k] = t;
+    }
+    return OMX_OK;
+}
+
 int omx_solver_set_policy(omx_solver *s, const float *params) {
     if (!s) return OMX_ERR_ARG;
     if (!params) {
@@ -557,6 +611,8 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
     s->timers.reset();
     PhaseTimers &T = s->timers;
     const double t0 = wall_ms();
+    OMX_HIP_CHECK(hipEventRecord(s->ev_start, cs));
+    s->ngrp_done = 0;
     if (stats) OMX_HIP_CHECK(hipMemsetAsync(s->feat.p, 0, s->feat.bytes, cs));
 
     // Factor panel i on the crit stream: getrf of the (up to date) diagonal
@@ -567,13 +623,16 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
     auto factor = [&](int64_t i) -> int {
         const int64_t ib = i * b, rest = n - ib - b;
         float *Akk = A + ib + ib * lda;
-        TIMED(T, kT_DIAG_LU, cs, 2.0 / 3.0 * b * (double)b * b, omx_getrf_nopiv_f32(b, Akk, lda, info + i, cs));
+        if (!(s->skip & OMX_SKIP_GETRF))
+            TIMED(T, kT_DIAG_LU, cs, 2.0 / 3.0 * b * (double)b * b, omx_getrf_nopiv_f32(b, Akk, lda, info + i, cs));
         int choice = 0;
         if (rest > 0) {
             float *Acol = A + (ib + b) + ib * lda;
             float *Arow = A + ib + (ib + b) * lda;
             const double rb2 = (double)rest * b * b;
-            if (i == 0 || s->alt == 0) {
+            if (s->skip & OMX_SKIP_TRSM) {
+                // panels left as they are (the reference's skip bit 2)
+            } else if (i == 0 || s->alt == 0) {
                 TIMED(T, kT_TRSM_L, cs, rb2, omx_trsm_f32(OMX_TRSM_RUNN, rest, b, Akk, lda, Acol, lda, cs));
                 TIMED(T, kT_TRSM_R, cs, rb2, omx_trsm_f32(OMX_TRSM_LLNU, b, rest, Akk, lda, Arow, lda, cs));
             } else {
@@ -630,7 +689,7 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
             OMX_HIP_CHECK(hipEventRecord(s->ev_la[t - h0], bs));
         }
         const int64_t f = G * (g + 2);
-        if (f < nb) {
+        if (f < nb && !(s->skip & OMX_SKIP_GEMM)) {
             const int64_t m = n - f * b;
             OMX_HIP_CHECK(hipEventRecord(s->ev_t0[s->bulk_count], bs));
             TIMED(T, kT_GEMM_UPDATE, bs, 2.0 * (double)m * m * (double)((i1 - i0 + 1) * b),
@@ -639,6 +698,9 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
             s->bulk_flops += 2.0 * (double)m * m * (double)((i1 - i0 + 1) * b);
             ++s->bulk_count;
         }
+        // every Schur product of group g's panels is done
+        OMX_HIP_CHECK(hipEventRecord(s->ev_grp[g], bs));
+        s->ngrp_done = g + 1;
         // ---- crit: the next group's panels ----
         // block t holds all earlier groups' updates once its look-ahead
         // products are done; the later blocks' products still run on the
@@ -650,6 +712,8 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
         }
         OMX_HIP_CHECK(hipEventRecord(s->ev_panel[g + 1], cs));
     }
+    // the last group's panels update only blocks of their own group (crit)
+    OMX_HIP_CHECK(hipEventRecord(s->ev_grp[s->ngrp_done], cs));
     OMX_HIP_CHECK(hipStreamSynchronize(cs));
     OMX_HIP_CHECK(hipStreamSynchronize(bs));
     s->t_lu_ms = wall_ms() - t0;
@@ -670,8 +734,9 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
         timings[3] = s->bulk_flops;
         timings[4] = s->crit_flops;
     }
-    for (int64_t i = 0; i < nb; ++i)
-        if (hinfo[i] != 0x7fffffff) return (int)(i * b + hinfo[i]);
+    if (!(s->skip & OMX_SKIP_GETRF))
+        for (int64_t i = 0; i < nb; ++i)
+            if (hinfo[i] != 0x7fffffff) return (int)(i * b + hinfo[i]);
     return OMX_OK;
 }
 
@@ -726,20 +791,16 @@ static int ir_solve_impl(omx_solver *s, const float *LU, int64_t lda, const omx_
                            std::isfinite(sys->beta) && sys->norm_inf > 0;
     if (sys->kind != OMX_SYSTEM_LCG && !lowrank_ok && !higham_ok) return OMX_ERR_ARG;
     const double *diag = sys->diag;
-    if (method == 1 && (restart <= 0 || restart > 512)) return OMX_ERR_ARG;
+    // only the two methods exist; GMRES's basis was sized by
+    // omx_solver_create (restart 50) or omx_solver_reserve_gmres: this call
+    // allocates nothing
+    if (method != 0 && method != 1) return OMX_ERR_ARG;
+    if (method == 1 && (restart <= 0 || restart > OMX_GMRES_RESTART_MAX)) return OMX_ERR_ARG;
+    if (method == 1 && restart > s->restart_cap) return OMX_ERR_WORKSPACE;
     const int64_t n = s->n, b = s->b;
     hipStream_t st = s->crit;
     double *r = static_cast<double *>(s->r.p), *w = static_cast<double *>(s->w.p);
     double *nrm = static_cast<double *>(s->nrm.p);
-    if (method == 1 && s->restart_cap < restart) {
-        dfree(s->V);
-        dfree(s->hv);
-        dfree(s->dots);
-        RC(dalloc(s->V, (size_t)(restart + 1) * n * 8));
-        RC(dalloc(s->hv, (size_t)(restart + 2) * 8));
-        RC(dalloc(s->dots, (size_t)(restart + 2) * kDotBlocks * 8));
-        s->restart_cap = restart;
-    }
     double hn[4];
     RC(omx_norm_inf_f64(n, diag, nrm, st));
     RC(omx_norm_inf_f64(n, rhs, nrm + 1, st));

@@ -1,11 +1,16 @@
 // omxp_main.cpp -- command-line driver with the reference's surface
 // (main.cpp:85-252, :273-736): positional N B [P Q] and the flags -solv,
 // -alt, -log, -progress, -init, plus -prec (fp16 | fp8 | learned), -policy,
-// -ir, -seed, -maxit, -restart, -device, and --highammat (the Fasi-Higham
-// matrix family, main.cpp:239-240, highammgen.hpp:4-30).  One GPU, 1x1 grid.  With
-// -prec learned the MLP policy picks each step's Schur precision; if the
+// -ir, -seed, -maxit, -restart, -device, --highammat (the Fasi-Higham
+// matrix family, main.cpp:239-240, highammgen.hpp:4-30) and -randsvd kappa
+// (a conditioned matrix of 2-norm condition number kappa).  One GPU, 1x1
+// grid.  With -prec learned the MLP policy picks each step's Schur
+// precision; whenever FP8 steps were used (-prec fp8 or learned) and the
 // refinement then fails, the system is generated again and factored in FP16
-// (inside the timed region).  The timed region is LU + IR, generation
+// (inside the timed region).  -skip is the reference's bit-mask (IR, GETRF,
+// TRSM, GEMM; main.cpp:68-70); -progress k prints the reference's
+// projected-rate line for every k-th step (panel_trf_gdirect.hpp:1146-1161)
+// from per-step completion events, after the factorisation.  The timed region is LU + IR, generation
 // excluded (main.cpp:614-657); the rate is (2/3 n^3 + 3/2 n^2) / time
 // (main.cpp:663).
 #include <hip/hip_runtime.h>
@@ -37,6 +42,8 @@ struct Opts {
     // cond_inf is kappa (higham_mat_impl.cpp:130-149)
     int higham = 0;
     double cond = 1e3, rho = 0.5;  // kappa and alpha / beta (higham_mat_impl.cpp:143-148)
+    double randsvd = 0;            // -randsvd kappa: A = U diag(sigma) V^T, kappa_2(A) = kappa
+    int nrefl = 8;                 // Householder reflectors per orthogonal factor
     double inner_tol = 1e-10;
     unsigned long long seed = 42;
     std::string prec = "fp16";
@@ -48,7 +55,8 @@ void usage() {
         "usage: omxp N B [P Q] [-solv 0|1] [-alt 0|1|2] [-log 0|1] [-progress k] [-init 1]\n"
         "            [-prec fp16|fp8|learned] [-policy file] [-ir classic|gmres] [-seed s] [-maxit k]\n"
         "            [-restart m] [-device d] [-group 2|4] [-timer]\n"
-        "            [--highammat | --highammat-paper] [-cond kappa] [-rho r]\n");
+        "            [--highammat | --highammat-paper] [-cond kappa] [-rho r]\n"
+        "            [-randsvd kappa] [-nrefl k] [-skip mask]\n");
 }
 
 bool parse(int argc, char **argv, Opts &o) {
@@ -88,6 +96,8 @@ bool parse(int argc, char **argv, Opts &o) {
         else if (!std::strcmp(a, "--highammat-paper")) o.higham = 2;
         else if (!std::strcmp(a, "-cond")) o.cond = std::atof(next(a));
         else if (!std::strcmp(a, "-rho")) o.rho = std::atof(next(a));
+        else if (!std::strcmp(a, "-randsvd")) o.randsvd = std::atof(next(a));
+        else if (!std::strcmp(a, "-nrefl")) o.nrefl = std::atoi(next(a));
         else if (!std::strcmp(a, "-h") || !std::strcmp(a, "--help")) {
             usage();
             std::exit(0);
@@ -120,11 +130,18 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "omxp runs on one GPU: P x Q must be 1 x 1 (got %ld x %ld)\n", o.p, o.q);
         return 2;
     }
-    if (o.skip) {
-        std::fprintf(stderr, "-skip is not supported: every run does the whole LU and IR\n");
+    if (o.skip < 0 || o.skip > 15) {
+        std::fprintf(stderr, "-skip is a mask of bits 0-3 (IR, GETRF, TRSM, GEMM)\n");
         return 2;
     }
     if (o.init != 1) std::printf("\t-init %d: only the on-device generator (-init 1) exists; using it\n", o.init);
+    // the reference's default driver factors the diagonal block on the GPU
+    // whatever -solv says (panel_trf_gdirect.hpp:707-711)
+    if (o.solv != 1) std::printf("\t-solv %d: the diagonal block is always factored on the GPU; using -solv 1\n", o.solv);
+    if (o.randsvd != 0 && (!(o.randsvd >= 1.0) || o.higham || o.nrefl < 1 || o.nrefl > 8)) {
+        std::fprintf(stderr, "-randsvd needs kappa >= 1, 1 <= -nrefl <= 8 and no --highammat\n");
+        return 2;
+    }
     if (o.alt < 0 || o.alt > 2 || o.b <= 0 || o.b % 256) {
         std::fprintf(stderr, "need -alt in 0..2 and B a multiple of 256\n");
         return 2;
@@ -183,6 +200,16 @@ int main(int argc, char **argv) {
     std::printf("\tn=%ld b=%ld 1x1 grid, alt=%d, Schur precision %s, IR %s (maxit %d)\n", n, b, o.alt,
                 learned ? "learned FP8/FP16 per step" : (fp8 ? "FP8 e4m3 (pow2 tile scale)" : "FP16"),
                 o.ir ? "GMRES" : "classical", o.maxit);
+    if (o.skip) {
+        // main.cpp:440-446
+        if (o.skip & OMX_SKIP_IR) std::printf("\tSkipping IR\n");
+        if (o.skip & OMX_SKIP_GETRF) std::printf("\tSkipping GETRF\n");
+        if (o.skip & OMX_SKIP_TRSM) std::printf("\tSkipping TRSM\n");
+        if (o.skip & OMX_SKIP_GEMM) std::printf("\tSkipping GEMM\n");
+    }
+    if (o.randsvd != 0)
+        std::printf("\trandsvd matrix: kappa_2=%g, %d seeded reflectors per orthogonal factor\n", o.randsvd,
+                    o.nrefl);
     if (o.higham)
         std::printf("\tFasi-Higham matrix (%s): kappa=%g rho=%g alpha=%.6e beta=%.6e\n",
                     o.higham == 1 ? "L = I + alpha N, U = I + beta N" : "L = I - alpha N, U = I - beta N", o.cond,
@@ -202,16 +229,43 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "solver creation failed\n");
         return 1;
     }
+    if (o.restart > OMX_GMRES_RESTART_DEFAULT && o.ir == 1 && omx_solver_reserve_gmres(s, o.restart) != 0) {
+        std::fprintf(stderr, "cannot reserve a GMRES basis for restart %d\n", o.restart);
+        return 2;
+    }
+    if (omx_solver_set_skip(s, o.skip) != 0) return 1;
+    // randsvd: the factors of diag(sigma) + P Q^T, the FP64 row sums of |A|
+    const int rank = 2 * o.nrefl;
+    double *sg = nullptr, *P = nullptr, *Q = nullptr, *rowabs = nullptr;
+    void *rwork = nullptr;
+    size_t rwork_bytes = 0;
+    if (o.randsvd != 0) {
+        rwork_bytes = omx_randsvd_workspace_bytes(n);
+        CK(hipMalloc(&sg, sizeof(double) * n));
+        CK(hipMalloc(&P, sizeof(double) * n * rank));
+        CK(hipMalloc(&Q, sizeof(double) * n * rank));
+        CK(hipMalloc(&rowabs, sizeof(double) * n));
+        CK(hipMalloc(&rwork, rwork_bytes));
+        if (omx_randsvd_factors(n, o.seed, o.randsvd, o.nrefl, sg, P, Q, nullptr) != 0) {
+            std::fprintf(stderr, "randsvd factors failed\n");
+            return 1;
+        }
+    }
     auto generate = [&]() {
+        if (o.randsvd != 0) return omx_randsvd_f32(n, rank, sg, P, Q, A, lda, d, rowabs, rwork, rwork_bytes, nullptr);
         return o.higham ? omx_higham_matgen_f32(n, h_alpha, h_beta, A, lda, d, nullptr)
                         : omx_matgen_f32(n, o.seed, A, lda, d, nullptr);
     };
     omx_system sys{};
-    sys.kind = o.higham ? OMX_SYSTEM_HIGHAM : OMX_SYSTEM_LCG;
+    sys.kind = o.randsvd != 0 ? OMX_SYSTEM_LOWRANK : (o.higham ? OMX_SYSTEM_HIGHAM : OMX_SYSTEM_LCG);
     sys.seed = o.seed;
     sys.diag = d;
     sys.alpha = h_alpha;
     sys.beta = h_beta;
+    sys.rank = rank;
+    sys.sigma = sg;
+    sys.P = P;
+    sys.Q = Q;
     sys.norm_inf = o.higham ? omx_higham_norm_inf(n, h_alpha, h_beta) : 0.0;
     double t = now_s();
     if (generate() || omx_rhs_f64(n, o.seed, rhs, nullptr)) {
@@ -219,6 +273,12 @@ int main(int argc, char **argv) {
         return 1;
     }
     CK(hipDeviceSynchronize());
+    if (o.randsvd != 0) {
+        // ||A||_inf = max_i sum_j |A_ij| (FP64, from the fill)
+        std::vector<double> ra(n);
+        CK(hipMemcpy(ra.data(), rowabs, sizeof(double) * n, hipMemcpyDeviceToHost));
+        sys.norm_inf = *std::max_element(ra.begin(), ra.end());
+    }
     std::printf("\tInitialization = %.4f sec\n", now_s() - t);
 
     std::vector<int> sched(n / b, fp8 ? 1 : 0);
@@ -237,34 +297,58 @@ int main(int argc, char **argv) {
         for (int v : used) n8 += v > 0;
         std::printf("learned schedule: %d of %ld steps in FP8\n", n8, (long)(n / b - 1));
     }
+    if (o.progress > 0) {
+        // the reference's projected-rate line (panel_trf_gdirect.hpp:1146-1161)
+        std::vector<double> ms(n / b, 0.0);
+        if (omx_solver_step_times(s, ms.data()) == 0) {
+            const double dn = (double)n;
+            for (long k = 0; k < n / b; k += o.progress) {
+                const double fact = 1.0 - (double)(k * b) / dn;
+                const double left = 100.0 * fact * fact * fact;
+                const double used_s = ms[k] * 1e-3;
+                const double proj_l = left < 100.0 ? used_s / (100.0 - left) * left : 0.0;
+                const double proj_t = used_s + proj_l;
+                const double gf = proj_t > 0 ? 2.0 / 3.0 * dn * dn * dn / proj_t * 1.0e-09 : 0.0;
+                std::printf("Step %ld, Rank 0, Elapsed %.2lf, Work left %.2lf%%, Time left: %.2lf, Est Total: "
+                            "%.2lf, GFlops: %.2lf, TFlops/GPU: %.2lf\n",
+                            k, used_s, left, proj_l, proj_t, gf, gf * 0.001);
+            }
+        }
+    }
     std::printf("LU factorization %.2f sec (trailing GEMM %.2f sec over %d launches, %.1f TFLOP/s)\n", t_lu,
                 tim[1] * 1e-3, (int)tim[2], tim[1] > 0 ? tim[3] / (tim[1] * 1e-3) * 1e-12 : 0.0);
     if (info) std::printf("LU reported a zero pivot at %d\n", info);
-    std::printf("\nIR Start with GPU\n");
     const double start_ir = now_s();
-    double out[4] = {0};
+    double out[4] = {-1.0, -1.0, 0.0, 0.0};
     std::vector<double> trace(2 * o.maxit, 0.0);
-    int rc = omx_ir_solve_system(s, A, lda, &sys, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, 16.0, 0, out,
+    const bool do_ir = !(o.skip & OMX_SKIP_IR);
+    int rc = 0;
+    if (do_ir) {
+        std::printf("\nIR Start with GPU\n");
+        rc = omx_ir_solve_system(s, A, lda, &sys, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, 16.0, 0, out,
                                  trace.data());
-    if (learned && rc != 0) {
-        // fallback: the learned schedule's factors did not refine; FP16
-        std::printf("learned schedule did not converge: regenerating and factoring in FP16\n");
+    }
+    if (do_ir && fp8 && rc != 0) {
+        // fallback: the factors with FP8 Schur steps did not refine; the
+        // system again, all steps in FP16
+        std::printf("FP8 schedule did not converge: regenerating and factoring in FP16\n");
         if (generate()) return 1;
         CK(hipDeviceSynchronize());
         std::fill(sched.begin(), sched.end(), 0);
         info = omx_lu_factor(s, A, lda, sched.data(), tim);
         rc = omx_ir_solve_system(s, A, lda, &sys, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, 16.0, 0, out,
                                  trace.data());
+        std::printf("fallback: FP16 factorisation, refinement %s\n", rc == 0 ? "converged" : "did not converge");
     }
     const double stop = now_s();
-    const int steps = (rc == 0) ? (int)out[2] + 1 : o.maxit;
+    const int steps = !do_ir ? 0 : (rc == 0) ? (int)out[2] + 1 : o.maxit;
     for (int i = 0; i < steps; ++i)
         std::printf("# iterative refinement: step=%d, residual=%e, hpl-harness=%f\n", i, trace[2 * i],
                     trace[2 * i + 1]);
     std::printf("\t IR Time %.2f sec\n", stop - start_ir);
     const double dur = stop - start;
     const double flops = 2.0 / 3.0 * (double)n * n * n + 3.0 / 2.0 * (double)n * n;
-    const double resid = rc == 0 ? out[0] : -1.0, harness = rc == 0 ? out[1] : -1.0;
+    const double resid = (do_ir && rc == 0) ? out[0] : -1.0, harness = (do_ir && rc == 0) ? out[1] : -1.0;
     std::printf("%.3f sec. %.3f GFlop/s resid = %20.15e hpl-harness = %.9f TFlop/s per GPU = %.3f\n", dur,
                 flops / dur * 1e-9, resid, harness, flops / dur * 1e-12);
     std::printf("===================================\n");
@@ -288,6 +372,8 @@ int main(int argc, char **argv) {
         }
     }
     omx_solver_destroy(s);
+    for (void *pp : {(void *)sg, (void *)P, (void *)Q, (void *)rowabs, rwork})
+        if (pp) (void)hipFree(pp);
     (void)hipFree(A);
     (void)hipFree(d);
     (void)hipFree(rhs);

@@ -417,12 +417,20 @@ void orc_trsm_llnu(int64_t m, int64_t n, const float *L, int64_t ldl, float *B,
  * unit-lower factor, solved on the identity (panel_trf_gdirect.hpp:121-125) */
 void orc_trsm_rlnu(int64_t m, int64_t n, const float *L, int64_t ldl, float *B,
                    int64_t ldb) {
+    /* element (i, j): b_ij -= L_lj b_il for l = j+1, ..., n-1 in that order,
+     * columns j from the last; done on 64-row column slices (contiguous in
+     * column-major storage), each element's terms in the same order */
+    const int64_t RB = 64;
 #pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < m; ++i) {
+    for (int64_t i0 = 0; i0 < m; i0 += RB) {
+        const int64_t i1 = (i0 + RB < m) ? i0 + RB : m;
         for (int64_t j = n - 1; j >= 0; --j) {
-            float bij = B[i + j * ldb];
-            for (int64_t l = j + 1; l < n; ++l) bij -= L[l + j * ldl] * B[i + l * ldb];
-            B[i + j * ldb] = bij;
+            float *bj = B + j * ldb;
+            for (int64_t l = j + 1; l < n; ++l) {
+                const float llj = L[l + j * ldl];
+                const float *bl = B + l * ldb;
+                for (int64_t i = i0; i < i1; ++i) bj[i] -= llj * bl[i];
+            }
         }
     }
 }
@@ -473,39 +481,79 @@ void orc_trtri_lower_unit(int64_t n, float *A, int64_t lda) {
 
 /* C = beta*C + alpha * X * Y^T with X (m x k), Yt (k x n, column j holds
  * Y(j, 0:k)) given as exact FP32 values; per element the k products are
- * summed in order in FP32 (shgemm, fp16sim.hpp:222-252).  The products of
- * FP16 (or scaled FP8) values are exact in FP32, so fused and unfused
- * multiply-add give the same sums. */
+ * summed in order in FP32 (shgemm, fp16sim.hpp:222-252), starting from 0,
+ * then combined with C.  The products of FP16 (or scaled FP8) values are
+ * exact in FP32, so fused and unfused multiply-add give the same sums.
+ *
+ * Full 16 x 6 tiles keep their sums in registers (X repacked into 16-row
+ * panels, contiguous in l); edge tiles take the plain loop.  Both add the
+ * same terms in the same order, so the tiling does not change any bit. */
+enum { GM_MR = 16, GM_NR = 6 };
+
+static void gemm_tile_store(int64_t mi, int64_t nj, const float *T, int64_t ldt, float alpha, float beta,
+                            float *C, int64_t ldc) {
+    for (int64_t c = 0; c < nj; ++c) {
+        float *cc = C + c * ldc;
+        const float *t = T + c * ldt;
+        if (beta == 0.0f)
+            for (int64_t i = 0; i < mi; ++i) cc[i] = t[i] * alpha;
+        else
+            for (int64_t i = 0; i < mi; ++i) cc[i] = cc[i] * beta + t[i] * alpha;
+    }
+}
+
 static void gemm_seq_core(int64_t m, int64_t n, int64_t k, float alpha,
                           const float *Xf, const float *Yt, float beta,
                           float *C, int64_t ldc) {
-    enum { IB = 128, JB = 8 };
-    const int64_t nib = (m + IB - 1) / IB, njb = (n + JB - 1) / JB;
+    const int64_t nib = (m + GM_MR - 1) / GM_MR, njb = (n + GM_NR - 1) / GM_NR;
+    /* Xp: 16-row panels, panel ib holds X(16 ib + r, l) at [(ib k + l) 16 + r]
+     * (rows past m are zero and never stored) */
+    float *Xp = (float *)malloc(sizeof(float) * (size_t)(nib * k * GM_MR + 1));
+#pragma omp parallel for schedule(static)
+    for (int64_t ib = 0; ib < nib; ++ib)
+        for (int64_t l = 0; l < k; ++l)
+            for (int64_t r = 0; r < GM_MR; ++r) {
+                const int64_t i = ib * GM_MR + r;
+                Xp[(ib * k + l) * GM_MR + r] = i < m ? Xf[i + l * m] : 0.0f;
+            }
+    /* row panels outermost: a thread's consecutive tiles share the panel,
+     * which stays in cache across the column blocks */
 #pragma omp parallel for collapse(2) schedule(static)
-    for (int64_t jb = 0; jb < njb; ++jb) {
-        for (int64_t ib = 0; ib < nib; ++ib) {
-            const int64_t i0 = ib * IB, j0 = jb * JB;
-            const int64_t mi = (i0 + IB <= m) ? IB : m - i0;
-            const int64_t nj = (j0 + JB <= n) ? JB : n - j0;
-            float T[JB][IB];
-            memset(T, 0, sizeof(T));
-            for (int64_t l = 0; l < k; ++l) {
-                const float *xl = Xf + l * m + i0;
-                for (int64_t c = 0; c < nj; ++c) {
-                    const float y = Yt[l + (j0 + c) * k];
-                    float *t = T[c];
-                    for (int64_t i = 0; i < mi; ++i) t[i] = __builtin_fmaf(xl[i], y, t[i]);
+    for (int64_t ib = 0; ib < nib; ++ib) {
+        for (int64_t jb = 0; jb < njb; ++jb) {
+            const int64_t i0 = ib * GM_MR, j0 = jb * GM_NR;
+            const int64_t mi = (i0 + GM_MR <= m) ? GM_MR : m - i0;
+            const int64_t nj = (j0 + GM_NR <= n) ? GM_NR : n - j0;
+            const float *xp = Xp + ib * k * GM_MR;
+            float T[GM_NR][GM_MR];
+            if (nj == GM_NR) {
+                float acc[GM_NR][GM_MR];
+                memset(acc, 0, sizeof(acc));
+                const float *y0 = Yt + j0 * k;
+                for (int64_t l = 0; l < k; ++l) {
+                    const float *xl = xp + l * GM_MR;
+#pragma GCC unroll 6
+                    for (int c = 0; c < GM_NR; ++c) {
+                        const float y = y0[l + c * k];
+#pragma GCC unroll 16
+                        for (int r = 0; r < GM_MR; ++r) acc[c][r] = __builtin_fmaf(xl[r], y, acc[c][r]);
+                    }
+                }
+                memcpy(T, acc, sizeof(T));
+            } else {
+                memset(T, 0, sizeof(T));
+                for (int64_t l = 0; l < k; ++l) {
+                    const float *xl = xp + l * GM_MR;
+                    for (int64_t c = 0; c < nj; ++c) {
+                        const float y = Yt[l + (j0 + c) * k];
+                        for (int r = 0; r < GM_MR; ++r) T[c][r] = __builtin_fmaf(xl[r], y, T[c][r]);
+                    }
                 }
             }
-            for (int64_t c = 0; c < nj; ++c) {
-                float *cc = C + (j0 + c) * ldc + i0;
-                if (beta == 0.0f)
-                    for (int64_t i = 0; i < mi; ++i) cc[i] = T[c][i] * alpha;
-                else
-                    for (int64_t i = 0; i < mi; ++i) cc[i] = cc[i] * beta + T[c][i] * alpha;
-            }
+            gemm_tile_store(mi, nj, &T[0][0], GM_MR, alpha, beta, C + i0 + j0 * ldc, ldc);
         }
     }
+    free(Xp);
 }
 
 /* C = beta*C + alpha * X * Y^T, X(i,l) = X[i + l*ldx] (m x k FP16),

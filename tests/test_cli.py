@@ -22,7 +22,9 @@ def test_help():
     assert r.returncode == 0 and "usage: omxp N B" in r.stdout
 
 
-@pytest.mark.parametrize("args", [("1024", "256", "2", "2"), ("1024", "256", "-skip", "1"),
+@pytest.mark.parametrize("args", [("1024", "256", "2", "2"), ("1024", "256", "-skip", "16"),
+                                  ("1024", "256", "-randsvd", "0.5"), ("1024", "256", "-randsvd", "100", "-nrefl", "9"),
+                                  ("1024", "256", "-randsvd", "100", "--highammat"),
                                   ("1024", "256", "-prec", "bf16"), ("1024", "100"), ("1024", "256", "-group", "3"),
                                   ("1024", "256", "-alt", "3"),
                                   ("1024", "256", "-prec", "learned", "-policy", "/nonexistent/p.f32"),
@@ -96,5 +98,68 @@ def test_highammat(extra):
     assert "Fasi-Higham matrix" in r.stdout
     if "fp8" in extra and r.returncode == 3:
         return  # FP8 factors that do not refine are reported, not hidden
+    assert r.returncode == 0, r.stderr
+    assert _harness(r.stdout) < 16.0
+
+
+@pytest.mark.gpu
+def test_skip_ir_and_progress():
+    """-skip 1 leaves out the refinement (main.cpp:637: no IR, the report
+    carries no residual); -progress 2 prints the reference's projected-rate
+    line for steps 0, 2, 4, 6 (panel_trf_gdirect.hpp:1146-1161)."""
+    from tests.gpu_util import torch_cuda
+    torch_cuda()
+    r = run(2048, 256, "-skip", "1", "-progress", "2")
+    print(r.stdout[-1200:], flush=True)
+    assert r.returncode == 0, r.stderr
+    assert "Skipping IR" in r.stdout and "IR Start" not in r.stdout
+    assert _harness(r.stdout) == -1.0
+    steps = [l for l in r.stdout.splitlines() if l.startswith("Step ")]
+    assert [int(l.split()[1].rstrip(",")) for l in steps] == [0, 2, 4, 6]
+    # elapsed time grows with the step; work left falls from 100 %
+    el = [float(l.split("Elapsed ")[1].split(",")[0]) for l in steps]
+    left = [float(l.split("Work left ")[1].split("%")[0]) for l in steps]
+    assert el == sorted(el) and left[0] == 100.0 and left == sorted(left, reverse=True)
+
+
+@pytest.mark.gpu
+def test_skip_gemm_breaks_the_factors():
+    """-skip 8 leaves out the Schur GEMMs (the reference's bit 3): the
+    factors are wrong and the refinement reports non-convergence (exit 3)."""
+    from tests.gpu_util import torch_cuda
+    torch_cuda()
+    r = run(2048, 256, "-skip", "8", "-maxit", "5")
+    print(r.stdout[-600:], flush=True)
+    assert "Skipping GEMM" in r.stdout
+    assert r.returncode == 3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kappa", ["1e2", "1e4"])
+def test_randsvd_fp16_solve(kappa):
+    """-randsvd kappa: the conditioned matrix diag(sigma) + P Q^T (8 seeded
+    reflectors per orthogonal factor) refines to harness < 16 in FP16."""
+    from tests.gpu_util import torch_cuda
+    torch_cuda()
+    r = run(2048, 256, "-randsvd", kappa, "-ir", "gmres")
+    print(r.stdout[-600:], flush=True)
+    assert "randsvd matrix: kappa_2=" in r.stdout
+    assert r.returncode == 0, r.stderr
+    assert _harness(r.stdout) < 16.0
+
+
+@pytest.mark.gpu
+def test_fp8_failure_falls_back_to_fp16():
+    """An FP8 factorisation that does not refine (randsvd, kappa 1e2: the
+    small singular values drown in e4m3's 3-bit mantissa) is followed, inside
+    the timed region, by the FP16 factorisation of the regenerated system,
+    which refines to harness < 16: the all-FP16 fallback of the learned
+    schedule."""
+    from tests.gpu_util import torch_cuda
+    torch_cuda()
+    r = run(2048, 256, "-randsvd", "1e2", "-prec", "fp8", "-ir", "gmres", "-maxit", "10")
+    print(r.stdout[-900:], flush=True)
+    assert "FP8 schedule did not converge" in r.stdout
+    assert "fallback: FP16 factorisation, refinement converged" in r.stdout
     assert r.returncode == 0, r.stderr
     assert _harness(r.stdout) < 16.0

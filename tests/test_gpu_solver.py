@@ -324,3 +324,27 @@ def test_phase_timers_off_by_default():
     assert L.omx_solver_timers(h, buf.ctypes.data, buf.ctypes.data) < 0
     L.omx_solver_destroy(h)
 
+
+
+def test_ir_rejects_unknown_method_and_unreserved_restart():
+    """omx_ir_solve checks its arguments before queueing any work: a method
+    other than 0 (classical) or 1 (GMRES) is OMX_ERR_ARG; a GMRES restart
+    above the basis omx_solver_create reserved (50) is OMX_ERR_WORKSPACE
+    until omx_solver_reserve_gmres enlarges it, after which it refines."""
+    torch = torch_cuda()
+    n, b, seed = 1024, 256, 11
+    A, d, rhs = _generate(n, seed, n)
+    S = Solver(n, b, 1)
+    info, _ = S.lu(A, n)
+    assert info == 0
+    x = empty(n, torch.float64)
+    for method in (2, -1, 7):
+        rc, out, _ = S.ir(A, n, seed, d, rhs, x, method=method)
+        assert rc == -1, (method, rc)
+    rc, out, _ = S.ir(A, n, seed, d, rhs, x, method=1, restart=60)
+    assert rc == -3
+    assert S.L.omx_solver_reserve_gmres(S.h, 60) == 0
+    assert S.L.omx_solver_reserve_gmres(S.h, 513) == -1
+    rc, out, _ = S.ir(A, n, seed, d, rhs, x, method=1, restart=60)
+    S.close()
+    assert rc == 0 and out[1] < 16.0
