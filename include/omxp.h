@@ -97,6 +97,22 @@ int omx_otf_gemv_f64(int64_t n, uint64_t seed, const double *diag,
                      double alpha, const double *x, double *y, void *work,
                      size_t work_bytes, void *stream);
 
+/* As omx_otf_gemv_f64 (same kernels, same column segments, bit-identical
+ * results) with the partial sums in the library's own static device array
+ * of 64 MiB instead of a workspace argument: it allocates nothing, for
+ * callers that cannot (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * 64 MiB cover every order n (the largest need, at n = 4194048, is 67104768
+ * bytes); larger needs would return OMX_ERR_WORKSPACE.  Every GPU holds its
+ * own copy from module load until the process ends; the first call on a
+ * device looks up its address (hipGetSymbolAddress), so make one call
+ * before any graph capture.  All calls on one GPU share the array: calls
+ * that overlap on one GPU (other streams or threads) race.  `stream` must
+ * belong to the current device. */
+int omx_otf_gemv_f64_noalloc(int64_t n, uint64_t seed, const double *diag,
+                             double alpha, const double *x, double *y,
+                             void *stream);
+
 /* *result (device) = max_i |x_i|.  Replaces calc_infnorm_d /
  * colv_infnorm_h (panel_norm.hpp:11-58). */
 int omx_norm_inf_f64(int64_t n, const double *x, double *result,

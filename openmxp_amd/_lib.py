@@ -29,6 +29,7 @@ SIGNATURES = {
     "omx_rhs_f64": (i32, [i64, u64, vp, vp]),
     "omx_otf_gemv_workspace_bytes": (sz, [i64]),
     "omx_otf_gemv_f64": (i32, [i64, u64, vp, dbl, vp, vp, vp, sz, vp]),
+    "omx_otf_gemv_f64_noalloc": (i32, [i64, u64, vp, dbl, vp, vp, vp]),
     "omx_norm_inf_f64": (i32, [i64, vp, vp, vp]),
     "omx_copy_f64": (i32, [i64, vp, vp, vp]),
     "omx_div_f64": (i32, [i64, vp, vp, vp, vp]),

@@ -3,7 +3,19 @@
 // the blocked triangular solves with the FP32 LU factors.
 #include <float.h>
 
+#include <mutex>
+
 #include "omx_common.hpp"
+
+// Partial sums of omx_otf_gemv_f64_noalloc: the device memory that entry
+// point uses in place of a caller's workspace.  64 MiB covers every order n
+// (the residual GEMV's largest need is two column segments at n = 4194048,
+// 67104768 bytes).  Each GPU the library runs on holds its own copy from
+// module load until the process ends.  Host code never passes this name to a
+// kernel (on the host it is only a stand-in for the symbol): it passes the
+// device address hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
 
 namespace {
 
@@ -391,14 +403,14 @@ size_t omx_otf_gemv_workspace_bytes(int64_t n) {
     return s > 1 ? (size_t)s * (size_t)n * sizeof(double) : 0;
 }
 
-int omx_otf_gemv_f64(int64_t n, uint64_t seed, const double *diag, double alpha,
-                     const double *x, double *y, void *work, size_t work_bytes,
-                     void *stream) {
-    if (n <= 0 || !diag || !x || !y) return OMX_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// The residual GEMV's launches; part holds omx_otf_gemv_workspace_bytes(n).
+int otf_gemv_launch(int64_t n, uint64_t seed, const double *diag, double alpha, const double *x, double *y,
+                    void *work, hipStream_t st) {
     const int nseg = gemv_segments(n);
-    const size_t need = omx_otf_gemv_workspace_bytes(n);
-    if (need > 0 && (!work || work_bytes < need)) return OMX_ERR_WORKSPACE;
-    hipStream_t st = omx_stream(stream);
     const int64_t seglen = omx_cdiv(n, nseg);
     const int64_t nrb = omx_cdiv(n, kGemvRows);
     double *part = nseg > 1 ? static_cast<double *>(work) : nullptr;
@@ -411,6 +423,48 @@ int omx_otf_gemv_f64(int64_t n, uint64_t seed, const double *diag, double alpha,
         OMX_LAUNCH_CHECK();
     }
     return OMX_OK;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (hipGetSymbolAddress with that device current) and kept.
+int noalloc_scratch(double **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return OMX_ERR_HIP;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return OMX_ERR_HIP;
+        g_scratch_dev[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return OMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int omx_otf_gemv_f64(int64_t n, uint64_t seed, const double *diag, double alpha,
+                     const double *x, double *y, void *work, size_t work_bytes,
+                     void *stream) {
+    if (n <= 0 || !diag || !x || !y) return OMX_ERR_ARG;
+    const size_t need = omx_otf_gemv_workspace_bytes(n);
+    if (need > 0 && (!work || work_bytes < need)) return OMX_ERR_WORKSPACE;
+    return otf_gemv_launch(n, seed, diag, alpha, x, y, work, omx_stream(stream));
+}
+
+int omx_otf_gemv_f64_noalloc(int64_t n, uint64_t seed, const double *diag, double alpha, const double *x,
+                             double *y, void *stream) {
+    if (n <= 0 || !diag || !x || !y) return OMX_ERR_ARG;
+    if (omx_otf_gemv_workspace_bytes(n) > kNoAllocScratchBytes) return OMX_ERR_WORKSPACE;
+    double *part = nullptr;
+    const int rc = noalloc_scratch(&part);
+    if (rc) return rc;
+    return otf_gemv_launch(n, seed, diag, alpha, x, y, part, omx_stream(stream));
 }
 
 int omx_norm_inf_f64(int64_t n, const double *x, double *result, void *stream) {
