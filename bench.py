@@ -72,6 +72,9 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=10240, help="order of the CPU baseline sample")
     ap.add_argument("--cpu-b", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ir-kernels", action="store_true", help="skip the triangular-solve / residual timings")
+    ap.add_argument("--randsvd-kappas", default="1e2,1e4,1e6",
+                    help="condition numbers of the randsvd legs (comma separated; empty: none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (the reference kernel for PMC byte counters)")
     ap.add_argument("--gemm-sweep", action="store_true",
@@ -206,6 +209,139 @@ def agg(res, flops):
                 steps_total=last["steps_total"], fell_back=any(r["fell_back"] for r in res))
 
 
+def randsvd_table(n, b, alt, kappas, policy, seed, group=4, nrefl=8):
+    """The conditioned systems: randsvd A = U diag(sigma) V^T (sigma
+    geometric from 1 to 1/kappa, U and V products of nrefl seeded Householder
+    reflectors) for each kappa, solved with the Schur update in FP16 on every
+    step, FP8 on every step, and the learned schedule (FP16 fallback when its
+    refinement fails), all with groups of `group` panels and GMRES-IR.  The
+    forward error is against the same solution refined on with three more
+    GMRES-IR corrections to the FP64 floor (residuals from the FP64 low-rank
+    description)."""
+    import numpy as np
+    import torch
+    from openmxp_amd import _lib, scheduler
+    L = _lib.lib()
+    flops = hpl_flops(n)
+    rows = []
+    for kappa in kappas:
+        sysm = scheduler.System("randsvd", n, seed, kappa=kappa, nrefl=nrefl)
+        x = torch.empty(n, dtype=torch.float64, device="cuda")
+        for meth in ("fp16", "fp8", "learned"):
+            if meth == "learned" and policy is None:
+                continue
+            h = C.c_void_p()
+            flags = _lib.SOLVER_FP8 | (_lib.SOLVER_GROUP4 if group == 4 else 0)
+            _lib.check(L.omx_solver_create(n, b, alt, flags, C.byref(h)), "solver create")
+            if meth == "learned":
+                pp = np.ascontiguousarray(policy, dtype=np.float32)
+                _lib.check(L.omx_solver_set_policy(h, pp.ctypes.data), "set policy")
+            nb = n // b
+            d = sysm.desc()
+
+            def lu_ir(sched):
+                sp = None
+                if sched is not None:
+                    sa = np.ascontiguousarray(sched, dtype=np.int32)
+                    sp = sa.ctypes.data
+                tim = np.zeros(5)
+                info = L.omx_lu_factor(h, sysm.A.data_ptr(), n, sp, tim.ctypes.data)
+                out = np.zeros(4)
+                rc = L.omx_ir_solve_system(h, sysm.A.data_ptr(), n, C.byref(d), sysm.rhs.data_ptr(), x.data_ptr(),
+                                           1, 50, 50, 1e-10, 16.0, 0, out.ctypes.data, None)
+                return info, rc, out, (sa if sched is not None else None)
+
+            sched = {"fp16": [0] * nb, "fp8": [1] * nb, "learned": None}[meth]
+            sysm.generate()
+            t0 = time.perf_counter()
+            info, rc, out, _ = lu_ir(sched)
+            torch.cuda.synchronize()
+            used = np.zeros(nb, np.int32)
+            L.omx_solver_last_schedule(h, used.ctypes.data, None)
+            fp8_steps = int((used[:-1] > 0).sum())
+            ok = info == 0 and rc == 0 and out[1] < 16.0
+            fell_back = False
+            if meth == "learned" and not ok:
+                fell_back = True
+                sysm.generate()
+                info, rc, out, _ = lu_ir([0] * nb)
+                torch.cuda.synchronize()
+                ok = info == 0 and rc == 0 and out[1] < 16.0
+            t = time.perf_counter() - t0
+            fwd = None
+            if info == 0:
+                xs = x.clone()
+                o2 = np.zeros(4)
+                rc2 = L.omx_ir_solve_system(h, sysm.A.data_ptr(), n, C.byref(d), sysm.rhs.data_ptr(), x.data_ptr(),
+                                            1, 3, 50, 1e-14, 0.0, 1, o2.ctypes.data, None)
+                torch.cuda.synchronize()
+                if rc2 == 0:
+                    fwd = float((xs - x).abs().max() / x.abs().max())
+            L.omx_solver_destroy(h)
+            row = {"kappa": kappa, "method": meth, "converged": bool(ok), "seconds": round(t, 3),
+                   "tflops": round(flops / t * 1e-12, 2), "harness": float(out[1]), "corrections": int(out[2]),
+                   "inner_iters": int(out[3]), "forward_error": fwd, "fp8_steps": fp8_steps, "steps": nb - 1}
+            if meth == "learned":
+                row["fell_back_to_fp16"] = fell_back
+            log(f"[randsvd kappa={kappa:g} {meth}] {t:.3f} s converged {ok} harness {out[1]:.4g} corrections "
+                f"{int(out[2])} inner {int(out[3])} forward error {fwd} FP8 steps {fp8_steps}/{nb - 1}"
+                + (" [fell back to FP16]" if fell_back else ""))
+            rows.append(row)
+        del sysm, x
+        torch.cuda.empty_cache()
+    return rows
+
+
+PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
+PEAK_FP64_VALU_TFLOPS = 78.6  # MI355X FP64 vector
+
+
+def ir_kernels(gs, seed, reps=3):
+    """The refinement's two HBM-bound triangular solves with the FP32 factors
+    (2 n^2 bytes of factor read each: half the n x n FP32 matrix) and its
+    VALU-bound residual GEMV (n^2 LCG values regenerated, n^2 FP64 FMAs),
+    each timed with HIP events on the null stream the C ABI launches on,
+    on the factored matrix in gs.A."""
+    import torch
+    L = gs.L
+    n, b = gs.n, gs.b
+    x = gs.rhs.clone()
+    y = torch.empty_like(x)
+    wb = L.omx_otf_gemv_workspace_bytes(n)
+    w = torch.empty(max(wb, 8) // 8, dtype=torch.float64, device="cuda")
+
+    def timed(fn):
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            assert fn() == 0
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        return min(ts)
+
+    tl = timed(lambda: L.omx_trsv_lower_unit_f32f64(n, b, gs.A.data_ptr(), gs.lda, x.data_ptr(), None, 0, None))
+    tu = timed(lambda: L.omx_trsv_upper_f32f64(n, b, gs.A.data_ptr(), gs.lda, x.data_ptr(), None, 0, None))
+    tg = timed(lambda: L.omx_otf_gemv_f64(n, seed, gs.d.data_ptr(), -1.0, x.data_ptr(), y.data_ptr(), w.data_ptr(),
+                                          wb, None))
+    tri = 2.0 * n * n
+    out = {}
+    for name, t in (("trsv_lower", tl), ("trsv_upper", tu)):
+        gbs = tri / t * 1e-9
+        out[name] = {"ms": round(t * 1e3, 3), "bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
+                     "bytes": tri, "launches": n // 256}
+    tf = 2.0 * n * n / tg * 1e-12
+    out["otf_gemv"] = {"ms": round(tg * 1e3, 3), "bound": "valu", "elements_per_s": round(n * n / tg, 1),
+                       "fp64_tflops": round(tf, 2), "peak_fp64_valu_tflops": PEAK_FP64_VALU_TFLOPS,
+                       "fp64_frac": round(tf / PEAK_FP64_VALU_TFLOPS, 4),
+                       "note": "each element is one LCG step (64-bit integer multiply-add), an int64->FP64 "
+                               "conversion and scaling, and one FP64 FMA: the FP64 fraction understates the "
+                               "VALU issue rate (see the SQ counter summary under profiles/)"}
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -290,9 +426,14 @@ def summarize_pmc(fetch_csv, write_csv, n, b):
           f"m = n - (2p+4) b), FP8 in groups of four (m x m x 4b, m = n - (4g+8) b)")
     print(f"fetch_calibration_factor: {cal['FETCH_SIZE']:.4f}  (bytes per counted byte, 1 GiB copy)")
     print(f"write_calibration_factor: {cal['WRITE_SIZE']:.4f}")
-    for tag, kname, eb in (("f16", "k_gemm_f16<false", 2.0), ("f8", "k_gemm_f8<false", 1.0)):
-        fetch = [v * cal["FETCH_SIZE"] for k, v in data["FETCH_SIZE"] if kname in k]
-        write = [v * cal["WRITE_SIZE"] for k, v in data["WRITE_SIZE"] if kname in k]
+    # the beta = 1 launches of either GEMM family (256 x 256 or 256 x 128 tiles)
+    families = {"f16": ("k_gemm_f16<false", "k_gemm_half<false, false"),
+                "f8": ("k_gemm_f8<false", "k_gemm_half<true, false")}
+    for tag, eb in (("f16", 2.0), ("f8", 1.0)):
+        names = families[tag]
+        fetch = [v * cal["FETCH_SIZE"] for k, v in data["FETCH_SIZE"] if any(f in k for f in names)]
+        write = [v * cal["WRITE_SIZE"] for k, v in data["WRITE_SIZE"] if any(f in k for f in names)]
+        kname = " / ".join(names)
         if not fetch or not write:
             continue
         nl = len(fetch)
@@ -300,7 +441,7 @@ def summarize_pmc(fetch_csv, write_csv, n, b):
         ms = bulk_shapes(n, b, G)
         # algorithmic bytes: FP32 C read + write, the two packed panels of K = G b
         alg = sum(8.0 * m * m + 2 * eb * m * G * b for m in ms)
-        print(f"{tag}.kernel: {kname},4> (trailing update)")
+        print(f"{tag}.kernel: {kname} (trailing update, beta = 1)")
         print(f"{tag}.launches: {nl} (algorithmic model: {len(ms)})")
         print(f"{tag}.fetch_bytes_total: {sum(fetch):.4e}")
         print(f"{tag}.write_bytes_total: {sum(write):.4e}")
@@ -416,6 +557,10 @@ def main():
     log(f"n={n} b={b} alt={args.alt}: generating and solving on cuda:0")
     gs = GpuSolve(n, b, args.alt, fp8=False, group=args.group_plain)
     res16, fwd16 = measure(gs, args, "plain fp16 + IR")
+    irk = None
+    if not args.no_ir_kernels:
+        irk = ir_kernels(gs, args.seed)
+        log("IR kernels: " + ", ".join(f"{k} {v['ms']} ms" for k, v in irk.items()))
     gs.close()
     del gs
     a16 = agg(res16, flops)
@@ -440,6 +585,10 @@ def main():
             a8 = agg(res8, flops)
         gs.close()
         del gs
+    rsvd = None
+    kappas = [float(v) for v in args.randsvd_kappas.split(",") if v.strip()]
+    if kappas:
+        rsvd = randsvd_table(n, b, args.alt, kappas, policy, args.seed, group=args.group)
     achieved = a16["gemm_flops"] / (a16["gemm_ms"] * 1e-3) * 1e-12
     cpu = gpu_small = None
     if not args.no_cpu:
@@ -449,7 +598,7 @@ def main():
     head = aL if aL else a16
     roof16 = {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
               "frac": round(achieved / PEAK_FP16_TFLOPS, 4), "traffic": pmc_traffic("f16", n, b, args.group_plain),
-              "kernel": f"k_gemm_f16 trailing update per {args.group_plain} panels (depth {args.group_plain}b, "
+              "kernel": f"FP16 Schur GEMM (omx_gemm_f16_packed_window) trailing update per {args.group_plain} panels (depth {args.group_plain}b, "
                         "alpha=-1, beta=1), plain run",
               "launches": a16["gemm_launches"]}
     line = {
@@ -489,7 +638,7 @@ def main():
         # the schedule chose FP8; all of them when fp8_steps == steps)
         line["roofline"] = {"bound": "mfma", "achieved": round(achL, 1), "peak": PEAK_FP8_TFLOPS, "unit": "TFLOP/s",
                             "frac": round(achL / PEAK_FP8_TFLOPS, 4), "traffic": pmc_traffic("f8", n, b, args.group),
-                            "kernel": f"k_gemm_f8 trailing update per {args.group} panels (depth {args.group}b, "
+                            "kernel": f"FP8 Schur GEMM (k_gemm_half, 256 x 128 tiles) trailing update per {args.group} panels (depth {args.group}b, "
                                       "alpha=-1, beta=1), learned run",
                             "launches": aL["gemm_launches"],
                             "fp8_steps": aL["fp8_steps"], "steps": aL["steps_total"]}
@@ -524,6 +673,11 @@ def main():
                                "policy_gain": round(aL["tflops"] / a8["tflops"], 3),
                                "note": "value / plain_fp16 = grouping_gain x fp8_gain x policy_gain; the two middle "
                                        "legs are timed over attr steps"}
+    if irk:
+        line["ir_kernels"] = irk
+    if rsvd:
+        line["randsvd"] = {"cases": rsvd, "n": n, "b": b, "group": args.group, "nrefl": 8, "ir": "GMRES-IR(50)",
+                           "forward_error_vs": "the same solution refined on with 3 more FP64 GMRES-IR corrections"}
     if cpu:
         line["cpu_baseline"] = {"value": round(cpu["value"], 5), "unit": "TFLOP/s", "cores": cpu["threads"],
                                 "kind": "port",

@@ -398,17 +398,38 @@ k_trsm_cols_llnu64(int64_t n, const float *__restrict__ L, int64_t ldl, float *_
 
 // One 64-column step of the blocked LU in one launch.  Every workgroup
 // factors the 64 x 64 diagonal block D at Ajj itself (wave 0, lu64_wave;
-// all get the same factors) into LDS; workgroup 0 also writes D back and
-// reports zero pivots.  Then workgroups [0, nrw) solve the rows below D
-// (RUNN with D's U), the others the columns right of D (LLNU with D's unit
-// L), 64 per workgroup, a quad per row or column; rest = rows below =
-// columns right of D.  Each quad's entries are loaded before the
-// factorisation, so the loads overlap it.
+// all get the same factors) into LDS; workgroup 0 reports zero pivots.
+// Then workgroups [0, nrw) solve the rows below D (RUNN with D's U), the
+// next nrw the columns right of D (LLNU with D's unit L), 64 per workgroup,
+// a quad per row or column; rest = rows below = columns right of D.  Each
+// quad's entries are loaded before the factorisation, so the loads overlap
+// it.
+// No workgroup writes D back in its own step: the workgroups of a launch
+// need not run at the same time (a concurrent kernel may hold the CUs), so
+// one that starts after D was overwritten would factor the factors.  The
+// factored D of the previous step (still the original in memory, and read
+// by no other workgroup of this launch) is written by one more workgroup,
+// the last one (index max(1, 2 nrw)), when prev is set; the last step's
+// workgroup 0, with nothing below or right of D, writes its own D.
 __global__ void __launch_bounds__(256)
-k_getrf_step64(int64_t rest, int nrw, float *__restrict__ Ajj, int64_t lda, int64_t row0, int *__restrict__ info) {
+k_getrf_step64(int64_t rest, int nrw, float *__restrict__ Ajj, int64_t lda, int64_t row0, int *__restrict__ info,
+               int prev) {
     __shared__ float d[kNB][kNB + 1];
     __shared__ float rd[kNB];
     const int tid = threadIdx.x, q = tid & 3;
+    if (prev && (int)blockIdx.x == (nrw > 0 ? 2 * nrw : 1)) {
+        // the previous step's diagonal block, 64 rows and columns up
+        if (tid < kNB) {
+            float *Dp = Ajj - kNB - kNB * lda;
+            float r[kNB];
+#pragma unroll
+            for (int j = 0; j < kNB; ++j) r[j] = Dp[tid + (int64_t)j * lda];
+            lu64_wave(r, tid, row0 - kNB, nullptr);
+#pragma unroll
+            for (int j = 0; j < kNB; ++j) Dp[tid + (int64_t)j * lda] = r[j];
+        }
+        return;
+    }
     const bool rows = (int)blockIdx.x < nrw;
     const int64_t e = (int64_t)(rows ? (int)blockIdx.x : (int)blockIdx.x - nrw) * 64 + (tid >> 2);
     const bool act = e < rest;
@@ -427,7 +448,7 @@ k_getrf_step64(int64_t rest, int nrw, float *__restrict__ Ajj, int64_t lda, int6
         lu64_wave(r, tid, row0, blockIdx.x == 0 ? info : nullptr);
 #pragma unroll
         for (int j = 0; j < kNB; ++j) d[tid][j] = r[j];
-        if (blockIdx.x == 0) {
+        if (nrw == 0) {
 #pragma unroll
             for (int j = 0; j < kNB; ++j) Ajj[tid + (int64_t)j * lda] = r[j];
         }
@@ -627,8 +648,9 @@ int omx_getrf_nopiv_f32(int64_t n, float *A, int64_t lda, int *info, void *strea
         float *Ajj = A + j + j * lda;
         const int64_t rest = n - j - kNB;
         const int nrw = (int)omx_cdiv(rest, 64);
-        hipLaunchKernelGGL(k_getrf_step64, dim3((unsigned)std::max(1, 2 * nrw)), dim3(256), 0, st, rest, nrw, Ajj, lda, j,
-                           info);
+        const int prev = j > 0 ? 1 : 0;  // one more workgroup writes the previous step's factored D
+        hipLaunchKernelGGL(k_getrf_step64, dim3((unsigned)(std::max(1, 2 * nrw) + prev)), dim3(256), 0, st, rest, nrw,
+                           Ajj, lda, j, info, prev);
         OMX_LAUNCH_CHECK();
         if (rest <= 0) break;
         int rc = sgemm(rest, rest, kNB, -1.0f, Ajj + kNB, lda, Ajj + kNB * lda, lda, 1.0f,

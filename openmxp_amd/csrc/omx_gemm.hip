@@ -18,6 +18,9 @@
 // The MFMA's A operand is the Y fragment and its B operand the X fragment,
 // so in the 32 x 32 result lane l holds row i = l%32 of C and 16 columns:
 // each store of a register writes 32 consecutive rows of one column.
+#include <stdlib.h>
+#include <string.h>
+
 #include <type_traits>
 
 #include "omx_common.hpp"
@@ -55,8 +58,11 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int &
     const int first = g * kGroupM;
     const int gm = min(tiles_m - first, kGroupM);
     const int in = logical - g * gsize;
-    tm = first + in % gm;
-    tn = in / gm;
+    // uniform values: the divisions above go through VALU reciprocals, and
+    // without the readfirstlane every buffer access whose descriptor derives
+    // from the tile would be wrapped in a waterfall loop
+    tm = __builtin_amdgcn_readfirstlane(first + in % gm);
+    tn = __builtin_amdgcn_readfirstlane(in / gm);
 }
 
 // s_waitcnt vmcnt(8 * stages): wait until at most `stages` later stages
@@ -132,27 +138,32 @@ __device__ __forceinline__ void epilogue(floatx16 (&acc)[4][4], int64_t M, int64
             }
             return;
         }
-        float old[2][4][16];
-        auto fetch = [&](float(&o)[4][16], int ni) {
+        // old values in units of two row blocks of one column block (32
+        // registers), the next unit's loads in flight while one is stored
+        float old[2][2][16];
+        auto fetch = [&](float(&o)[2][16], int u) {
+            const int ni = u >> 1, m0 = 2 * (u & 1);
             const __amdgpu_buffer_rsrc_t rs = colblock_rsrc(C + (j0 + 32 * ni) * ldc + i0, ldc);
 #pragma unroll
-            for (int mi = 0; mi < 4; ++mi)
+            for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int r = 0; r < 16; ++r)
-                    o[mi][r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff[r] + mi * 128, 0, kCPolicy));
+                    o[h][r] = __builtin_bit_cast(
+                        float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff[r] + (m0 + h) * 128, 0, kCPolicy));
         };
         fetch(old[0], 0);
 #pragma unroll
-        for (int ni = 0; ni < 4; ++ni) {
-            if (ni + 1 < 4) fetch(old[(ni + 1) & 1], ni + 1);
+        for (int u = 0; u < 8; ++u) {
+            if (u + 1 < 8) fetch(old[(u + 1) & 1], u + 1);
+            const int ni = u >> 1, m0 = 2 * (u & 1);
             const __amdgpu_buffer_rsrc_t rs = colblock_rsrc(C + (j0 + 32 * ni) * ldc + i0, ldc);
 #pragma unroll
-            for (int mi = 0; mi < 4; ++mi)
+            for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int r = 0; r < 16; ++r)
                     __builtin_amdgcn_raw_buffer_store_b32(
-                        __builtin_bit_cast(uint32_t, beta * old[ni & 1][mi][r] + alpha * acc[mi][ni][r]), rs,
-                        voff[r] + mi * 128, 0, kCPolicy);
+                        __builtin_bit_cast(uint32_t, beta * old[u & 1][h][r] + alpha * acc[m0 + h][ni][r]), rs,
+                        voff[r] + (m0 + h) * 128, 0, kCPolicy);
         }
         return;
     }
@@ -203,7 +214,8 @@ k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char 
            const int32_t *__restrict__ xsexp, const int32_t *__restrict__ ysexp, int64_t xstrip,
            int64_t ystrip) {
     __shared__ __attribute__((aligned(1024))) char smem[kNbuf][2][16384];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar tile offsets
     const int wm = wave & 1, wn = wave >> 1;
     int tm, tn;
     tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
@@ -320,7 +332,8 @@ k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *
           float *__restrict__ C, int64_t ldc, int tiles_m, int tiles_n, int64_t xstrip, int64_t ystrip,
           int64_t xet, int64_t yet) {
     __shared__ __attribute__((aligned(1024))) char smem[kNbuf][2][16384];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar tile offsets
     const int wm = wave & 1, wn = wave >> 1;
     int tm, tn;
     tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
@@ -432,6 +445,335 @@ k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *
     epilogue<kBetaZero>(acc, M, N, alpha, beta, C, ldc, tm, tn, wm, wn, lane);
 }
 
+// ---------------------------------------------------------------------------
+// Two workgroups per CU.  Workgroup tile 256 x 128: an X strip against one
+// half (128 rows) of a Y strip.  Four waves in a 2 x 2 arrangement, each
+// owning 128 x 64 of C as 4 x 2 accumulators of a 32 x 32 MFMA (128
+// registers), so that a wave fits in 256 registers and two workgroups share
+// a CU: while one waits on memory, passes a barrier or runs its C epilogue,
+// the other keeps the matrix pipes busy.
+//
+// LDS: a ring of kSlots slots of 12 KB (72 KB per workgroup).  Slot s holds,
+// for each of the 8 X and 4 Y fragment rows (32 matrix rows each), the 1 KB
+// block (s & 3) of packed chunk s >> 2: in FP16 the 32 x 16 fragment of
+// columns 16 s .. 16 s + 15, in FP8 one 1 KB half of a 32 x 64 fragment.  A
+// k-step (one MFMA depth: 16 FP16 or 64 FP8 columns) takes kP slots (1 or
+// 2); the ring holds kR = kSlots / kP k-steps.
+//
+// Step st: wait until this wave's copies of step st + 1 have landed and its
+// fragment reads of step st are in registers; barrier (so every wave's
+// copies of step st + 1 are in LDS and no wave still reads step st's
+// slots); refill step st's slots with step st + kR; read step st + 1's
+// fragments; run step st's 8 MFMAs.  A copy has kR - 1 steps to land.
+// ---------------------------------------------------------------------------
+constexpr int kSlots = 6;
+constexpr int kSlotBytes = 12288;
+
+// s_waitcnt vmcnt(v) lgkmcnt(0) (expcnt not waited), v < 64
+template <int v>
+__device__ __forceinline__ void wait_vm_lgkm0() {
+    static_assert(v >= 0 && v < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((v & 15) | ((v >> 4) << 14) | (7 << 4));
+}
+
+// wait until at most c k-steps of copies (kLoads per wave each) are in
+// flight and every LDS read has returned
+template <int kLoads>
+__device__ __forceinline__ void wait_steps(int c) {
+    switch (c) {
+        case 0: wait_vm_lgkm0<0>(); break;
+        case 1: wait_vm_lgkm0<kLoads>(); break;
+        case 2: wait_vm_lgkm0<2 * kLoads>(); break;
+        case 3: wait_vm_lgkm0<3 * kLoads>(); break;
+        case 4: wait_vm_lgkm0<4 * kLoads>(); break;
+        default: wait_vm_lgkm0<5 * kLoads>(); break;
+    }
+}
+
+// C update of one wave's 128 x 64 block: acc[mi][ni][r] is
+//   C(i0 + 32 mi + lane%32, j0 + 32 ni + 8 (r/4) + 4 (lane/32) + r%4).
+// Interior tiles: buffer instructions, one resource per 32-column block; the
+// old values move in units of two row blocks of one column block (32
+// registers), the next unit's loads in flight while one is combined and
+// stored.
+template <bool kBetaZero>
+__device__ __forceinline__ void epilogue_half(floatx16 (&acc)[4][2], int64_t M, int64_t N, float alpha, float beta,
+                                              float *__restrict__ C, int64_t ldc, int tm, int tn, int wm, int wn,
+                                              int lane) {
+    const int64_t i0 = (int64_t)tm * OMX_STRIP + wm * 128;
+    const int64_t j0 = (int64_t)tn * 128 + wn * 64;
+    if ((int64_t)(tm + 1) * OMX_STRIP <= M && (int64_t)(tn + 1) * 128 <= N && ldc * 32 * 4 < ((int64_t)1 << 31)) {
+        int voff[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            voff[r] = (int)(((8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * ldc + (lane & 31)) * 4);
+        if (kBetaZero) {
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni) {
+                const __amdgpu_buffer_rsrc_t rs = colblock_rsrc(C + (j0 + 32 * ni) * ldc + i0, ldc);
+#pragma unroll
+                for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, alpha * acc[mi][ni][r]),
+                                                              rs, voff[r] + mi * 128, 0, kCPolicy);
+            }
+            return;
+        }
+        float old[2][2][16];
+        auto fetch = [&](float(&o)[2][16], int u) {
+            const int ni = u >> 1, m0 = 2 * (u & 1);
+            const __amdgpu_buffer_rsrc_t rs = colblock_rsrc(C + (j0 + 32 * ni) * ldc + i0, ldc);
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    o[h][r] = __builtin_bit_cast(
+                        float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff[r] + (m0 + h) * 128, 0, kCPolicy));
+        };
+        fetch(old[0], 0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (u + 1 < 4) fetch(old[(u + 1) & 1], u + 1);
+            const int ni = u >> 1, m0 = 2 * (u & 1);
+            const __amdgpu_buffer_rsrc_t rs = colblock_rsrc(C + (j0 + 32 * ni) * ldc + i0, ldc);
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    __builtin_amdgcn_raw_buffer_store_b32(
+                        __builtin_bit_cast(uint32_t, beta * old[u & 1][h][r] + alpha * acc[m0 + h][ni][r]), rs,
+                        voff[r] + (m0 + h) * 128, 0, kCPolicy);
+        }
+        return;
+    }
+    // edge tiles (and ldc too large for 32-bit column offsets): checked
+    const int64_t ib = i0 + (lane & 31);
+    const int64_t jb = j0 + 4 * (lane >> 5);
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+        const int64_t i = ib + mi * 32;
+        if (i >= M) continue;
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) {
+            float *cp = C + i;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t j = jb + ni * 32 + 8 * (r >> 2) + (r & 3);
+                if (j < N) {
+                    const float v = acc[mi][ni][r];
+                    cp[j * ldc] = kBetaZero ? alpha * v : beta * cp[j * ldc] + alpha * v;
+                }
+            }
+        }
+    }
+}
+
+template <bool kMore_, bool kIssue_, int kWait_>
+struct StepCfg {
+    static constexpr bool kMore = kMore_;
+    static constexpr bool kIssue = kIssue_;
+    static constexpr int kWait = kWait_;
+};
+
+// The last kT steps (kT even) from step s0, alternating the two fragment
+// sets; step t's flags from its position.
+template <int kT, int kR, int t = 0, class F, class XA, class YA>
+__device__ __forceinline__ void tail_steps(F &step, XA &xa, YA &ya, XA &xb, YA &yb, int s0) {
+    if constexpr (t < kT) {
+        constexpr int kWait = (kR - 2 < kT - t - 2) ? kR - 2 : kT - t - 2;
+        typedef StepCfg<(t < kT - 1), (t + kR < kT), (kWait < 0 ? 0 : kWait)> Cfg;
+        if constexpr (t % 2 == 0)
+            step(xa, ya, xb, yb, s0 + t, Cfg{});
+        else
+            step(xb, yb, xa, ya, s0 + t, Cfg{});
+        tail_steps<kT, kR, t + 1>(step, xa, ya, xb, yb, s0);
+    }
+}
+
+// Step depth and fragment buffering of the half-tile kernels: kP slots per
+// k-step (FP16: 1 or 2 16-column MFMA depths per step; FP8: always 2, the
+// two halves of one 64-column operand); kDB: the next step's fragments in
+// their own registers, read between the current step's MFMAs (double
+// buffering), or (0) each fragment row read into the registers of the one
+// it replaces once that one's last MFMA has issued.
+#ifndef OMX_HALF_F16_P
+#define OMX_HALF_F16_P 1
+#endif
+#ifndef OMX_HALF_F16_DB
+#define OMX_HALF_F16_DB 1
+#endif
+#ifndef OMX_HALF_F8_DB
+#define OMX_HALF_F8_DB 1
+#endif
+
+template <bool kFp8, bool kBetaZero, int kP, bool kDB>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_gemm_half(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *__restrict__ X,
+            const int32_t *__restrict__ xexp, const char *__restrict__ Y, const int32_t *__restrict__ yexp,
+            float *__restrict__ C, int64_t ldc, int tiles_m, int tiles_n, int64_t xstrip, int64_t ystrip,
+            int64_t xet, int64_t yet) {
+    static_assert(!kFp8 || kP == 2, "an FP8 operand spans two slots");
+    constexpr int kR = kSlots / kP;                    // k-steps in the ring
+    constexpr int kCols = (kFp8 ? 32 : 16) * kP;       // columns per k-step
+    constexpr int kLoads = 3 * kP;                     // copies per wave per k-step
+    typedef typename std::conditional<kP == 1, intx4, intx8>::type Frag;
+    __shared__ __attribute__((aligned(1024))) char smem[kSlots * kSlotBytes];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar tile offsets
+    const int wm = wave & 1, wn = wave >> 1;
+    int tm, tn;
+    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const int nsteps = (int)(K / kCols);
+    const int ys_strip = tn >> 1;
+    // this wave's three 1 KB blocks of every slot: block b = 3 wave + t is X
+    // fragment row b (b < 8) or Y fragment row 4 (tn & 1) + b - 8
+    const char *src[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const int b = 3 * wave + t;
+        src[t] = (b < 8 ? X + (int64_t)tm * xstrip + b * 4096
+                        : Y + (int64_t)ys_strip * ystrip + (4 * (tn & 1) + b - 8) * 4096) + lane * 16;
+    }
+    const int32_t *xe = kFp8 ? xexp + (int64_t)tm * xet : nullptr;
+    const int32_t *ye = kFp8 ? yexp + (int64_t)ys_strip * yet : nullptr;
+
+    // k-step `step` is slots kP step .. kP step + kP - 1
+    auto issue = [&](int step) {
+#pragma unroll
+        for (int p = 0; p < kP; ++p) {
+            const int s = kP * step + p;
+            const int64_t off = (int64_t)(s >> 2) * kChunk + (s & 3) * 1024;
+            char *l = smem + (s % kSlots) * kSlotBytes;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) gl_lds16(src[t] + off, l + (3 * wave + t) * 1024);
+        }
+    };
+    // A lane's operand of one k-step for fragment row f (0-7 X, 8-11 Y): 16
+    // bytes from each of the step's slots.  FP16: one 16-column MFMA operand
+    // per slot; FP8: the 32 bytes of one 64-column MFMA operand.
+    auto frag = [&](int step, int f) -> Frag {
+        const char *l0 = smem + ((kP * step) % kSlots) * kSlotBytes + f * 1024 + lane * 16;
+        const intx4 lo = *reinterpret_cast<const intx4 *>(l0);
+        if constexpr (kP == 1) {
+            return lo;
+        } else {
+            const intx4 hi = *reinterpret_cast<const intx4 *>(l0 + kSlotBytes);
+            return intx8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        }
+    };
+    auto mfma = [&](floatx16 &c, const Frag &y, const Frag &x, int sy, int sx) {
+        if constexpr (kFp8) {
+            c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(y, x, c, 0, 0, 0, sy, 0, sx);
+        } else if constexpr (kP == 1) {
+            c = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(half8, y), __builtin_bit_cast(half8, x), c,
+                                                       0, 0, 0);
+        } else {
+            const intx4 ylo{y[0], y[1], y[2], y[3]}, yhi{y[4], y[5], y[6], y[7]};
+            const intx4 xlo{x[0], x[1], x[2], x[3]}, xhi{x[4], x[5], x[6], x[7]};
+            c = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(half8, ylo), __builtin_bit_cast(half8, xlo),
+                                                       c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(half8, yhi), __builtin_bit_cast(half8, xhi),
+                                                       c, 0, 0, 0);
+        }
+    };
+
+    floatx16 acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = floatx16{};
+
+    for (int st = 0; st < kR && st < nsteps; ++st) issue(st);
+    wait_steps<kLoads>(min(kR - 1, nsteps - 1));
+    stage_barrier();
+    Frag xa[4], ya[2], xb[4], yb[2];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) xa[q] = frag(0, wm * 4 + q);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) ya[q] = frag(0, 8 + wn * 2 + q);
+
+    // One k-step.  Cfg::kMore: a later step exists (its fragments are read);
+    // Cfg::kIssue: step st + kR exists and is copied into step st's slots;
+    // Cfg::kWait: k-steps of copies that may stay in flight at the wait.
+    auto step = [&](Frag(&xc)[4], Frag(&yc)[2], Frag(&xn)[4], Frag(&yn)[2], int st, auto cfg) {
+        typedef decltype(cfg) Cfg;
+        constexpr bool more = Cfg::kMore;
+        if (more) {
+            wait_steps<kLoads>(Cfg::kWait);
+            stage_barrier();
+            if (Cfg::kIssue) issue(st + kR);
+        }
+        int sx = 0, sy = 0;
+        if constexpr (kFp8) {
+            const int kt = (st * 64) / OMX_SCALE_TILE;
+            sx = 127 - xe[kt];
+            sy = 127 - ye[kt];
+        }
+        if constexpr (kDB) {
+            if (more) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) xn[q] = frag(st + 1, wm * 4 + q);
+#pragma unroll
+                for (int q = 0; q < 2; ++q) yn[q] = frag(st + 1, 8 + wn * 2 + q);
+            }
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni) mfma(acc[mi][ni], yc[ni], xc[mi], sy, sx);
+            if (more && Cfg::kIssue) {
+                // MFMA instructions, LDS reads and copies of the step spread
+                // evenly: 8 (FP8, FP16 depth 16) or 16 MFMAs, 6 or 12 reads,
+                // 3 or 6 copies
+                constexpr int kM = (kFp8 || kP == 1) ? 8 : 16, kD = 6 * kP, kV = 3 * kP;
+                constexpr int kG = kV;
+#pragma unroll
+                for (int g = 0; g < kG; ++g) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, kM / kG, 0);  // MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x100, kD / kG, 0);  // DS read
+                    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);        // VMEM
+                }
+            }
+        } else {
+            // the next step's X fragment of row mi is read once the current
+            // one's last MFMA has issued, its Y fragments after the last
+            // MFMA (fragments in 2/3 of the double-buffered registers)
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi) {
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni) mfma(acc[mi][ni], yc[ni], xc[mi], sy, sx);
+                if (more) xn[mi] = frag(st + 1, wm * 4 + mi);
+            }
+            if (more) {
+                yn[0] = frag(st + 1, 8 + wn * 2);
+                yn[1] = frag(st + 1, 8 + wn * 2 + 1);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    // nsteps is even.  Steady state: pairs of steps with every flag set;
+    // then the last kT steps with their flags fixed at compile time (tail
+    // step t: a later step if t < kT - 1, a refill if t + kR < kT), so
+    // neither part branches.  FP8 products have at least 4 steps (K % 256
+    // == 0); shorter FP16 products (K = 64: 4 steps of 16 or 2 of 32) take
+    // a static tail of their own length.
+    constexpr int kT = kR + 1 + ((kR + 1) & 1);  // even, > kR
+    if (kFp8 || nsteps >= kT) {
+        int st = 0;
+        for (; st + kT < nsteps; st += 2) {
+            step(xa, ya, xb, yb, st, StepCfg<true, true, kR - 2>{});
+            step(xb, yb, xa, ya, st + 1, StepCfg<true, true, kR - 2>{});
+        }
+        tail_steps<kT, kR>(step, xa, ya, xb, yb, nsteps - kT);
+    } else if constexpr (!kFp8) {
+        constexpr int kShort = 64 / kCols;  // K = 64
+        tail_steps<kShort, kR>(step, xa, ya, xb, yb, 0);
+    }
+    float a = alpha;
+    if (!kFp8 && xexp) a = ldexpf(alpha, -(xexp[tm] + yexp[ys_strip]));
+    epilogue_half<kBetaZero>(acc, M, N, a, beta, C, ldc, tm, tn, wm, wn, lane);
+}
+
 constexpr int kNbuf = OMX_GEMM_NBUF;
 
 int gemm_args(int64_t m, int64_t n, int64_t k, int64_t kstage, const void *X, int64_t xr0,
@@ -441,6 +783,13 @@ int gemm_args(int64_t m, int64_t n, int64_t k, int64_t kstage, const void *X, in
     const int64_t tiles = omx_cdiv(m, OMX_STRIP) * omx_cdiv(n, OMX_STRIP);
     if (tiles >= ((int64_t)1 << 31)) return OMX_ERR_ARG;
     return OMX_OK;
+}
+
+// The trailing-update kernel family: 256 x 128 tiles two workgroups per CU
+// (default) or 256 x 256 tiles one per CU (OMX_GEMM_TILE=256).
+bool use_half_tiles() {
+    const char *e = getenv("OMX_GEMM_TILE");
+    return !(e && !strcmp(e, "256"));
 }
 
 // the K window [k0, k0 + k) of a packed panel of ldk columns
@@ -467,6 +816,21 @@ int omx_gemm_f16_packed_window(int64_t m, int64_t n, int64_t k, float alpha, con
     const char *yp = reinterpret_cast<const char *>(Y) + (y_row0 / OMX_STRIP) * ystrip + (y_k0 / OMX_KSTAGE16) * kChunk;
     const int32_t *xe = xsexp ? xsexp + x_row0 / OMX_STRIP : nullptr;
     const int32_t *ye = ysexp ? ysexp + y_row0 / OMX_STRIP : nullptr;
+    if (use_half_tiles()) {
+        const int tiles_h = (int)omx_cdiv(n, 128);
+        if ((int64_t)tiles_m * tiles_h >= ((int64_t)1 << 31)) return OMX_ERR_ARG;
+        dim3 grid((unsigned)(tiles_m * tiles_h));
+        if (beta == 0.0f)
+            hipLaunchKernelGGL((k_gemm_half<false, true, OMX_HALF_F16_P, OMX_HALF_F16_DB>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k,
+                               alpha, beta, xp, xe, yp, ye, C, ldc, tiles_m, tiles_h, xstrip, ystrip, (int64_t)0,
+                               (int64_t)0);
+        else
+            hipLaunchKernelGGL((k_gemm_half<false, false, OMX_HALF_F16_P, OMX_HALF_F16_DB>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k,
+                               alpha, beta, xp, xe, yp, ye, C, ldc, tiles_m, tiles_h, xstrip, ystrip, (int64_t)0,
+                               (int64_t)0);
+        OMX_LAUNCH_CHECK();
+        return OMX_OK;
+    }
     dim3 grid((unsigned)(tiles_m * tiles_n));
     if (beta == 0.0f)
         hipLaunchKernelGGL((k_gemm_f16<true, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
@@ -508,6 +872,19 @@ int omx_gemm_f8_packed_window(int64_t m, int64_t n, int64_t k, float alpha, cons
     const char *yp = reinterpret_cast<const char *>(Y) + (y_row0 / OMX_STRIP) * ystrip + (y_k0 / OMX_KSTAGE8) * kChunk;
     const int32_t *xe = xexp + (x_row0 / OMX_STRIP) * xet + x_k0 / OMX_SCALE_TILE;
     const int32_t *ye = yexp + (y_row0 / OMX_STRIP) * yet + y_k0 / OMX_SCALE_TILE;
+    if (use_half_tiles()) {
+        const int tiles_h = (int)omx_cdiv(n, 128);
+        if ((int64_t)tiles_m * tiles_h >= ((int64_t)1 << 31)) return OMX_ERR_ARG;
+        dim3 grid((unsigned)(tiles_m * tiles_h));
+        if (beta == 0.0f)
+            hipLaunchKernelGGL((k_gemm_half<true, true, 2, OMX_HALF_F8_DB>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k,
+                               alpha, beta, xp, xe, yp, ye, C, ldc, tiles_m, tiles_h, xstrip, ystrip, xet, yet);
+        else
+            hipLaunchKernelGGL((k_gemm_half<true, false, 2, OMX_HALF_F8_DB>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k,
+                               alpha, beta, xp, xe, yp, ye, C, ldc, tiles_m, tiles_h, xstrip, ystrip, xet, yet);
+        OMX_LAUNCH_CHECK();
+        return OMX_OK;
+    }
     dim3 grid((unsigned)(tiles_m * tiles_n));
     if (beta == 0.0f)
         hipLaunchKernelGGL((k_gemm_f8<true, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,

@@ -63,7 +63,7 @@ class System:
     """One generated system: A (FP32, device), its FP64 diagonal, rhs, and
     for randsvd the low-rank description."""
 
-    def __init__(self, kind: str, n: int, seed: int, kappa: float = 1.0, nrefl: int = 2):
+    def __init__(self, kind: str, n: int, seed: int, kappa: float = 1.0, nrefl: int = 8):
         import torch
         self.torch = torch
         L = _lib.lib()
@@ -114,6 +114,7 @@ class Solver:
         self.h = C.c_void_p()
         _lib.check(L.omx_solver_create(n, b, alt, 1, C.byref(self.h)), "solver create")
         self.n, self.b, self.nb = n, b, n // b
+        self.solves = 0  # factorisations + refinements run by solve()
 
     def set_policy(self, params):
         L = _lib.lib()
@@ -132,6 +133,7 @@ class Solver:
         L = _lib.lib()
         if regenerate:
             system.generate()
+        self.solves += 1
         n = self.n
         x = torch.empty(n, dtype=torch.float64, device="cuda")
         sp = None
@@ -265,12 +267,17 @@ def train(F, y, t, m, iters: int = 3000, lr: float = 3e-3, seed: int = 0, log=pr
     return P.cpu().numpy(), losses, acc
 
 
-def training_systems(count: int, seed: int = 1000):
-    """Seeded training systems: LCG and randsvd, n in {8192, 16384}."""
+TRAIN_SIZES = (8192, 16384, 32768)
+
+
+def training_systems(count: int, seed: int = 1000, sizes=TRAIN_SIZES):
+    """Seeded training systems: every third the diagonally dominant LCG
+    matrix, the others randsvd with kappa log-uniform in [1e1, 1e6]; n drawn
+    from `sizes`."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
-        n = int(rng.choice([8192, 16384]))
+        n = int(rng.choice(list(sizes)))
         if i % 3 == 0:
             out.append(("lcg", n, seed + i, 1.0))
         else:
@@ -281,41 +288,68 @@ def training_systems(count: int, seed: int = 1000):
 def main(argv=None):
     ap = argparse.ArgumentParser(description="train the learned FP8/FP16 schedule on the GPU")
     ap.add_argument("cmd", choices=["train"])
-    ap.add_argument("--systems", type=int, default=36)
+    ap.add_argument("--systems", type=int, default=400, help="most systems to label")
+    ap.add_argument("--solves", type=int, default=2000,
+                    help="stop labelling new systems once this many solves have run")
+    ap.add_argument("--sizes", default=",".join(map(str, TRAIN_SIZES)))
     ap.add_argument("--b", type=int, default=512)
     ap.add_argument("--slack", type=int, default=3)
     ap.add_argument("--iters", type=int, default=3000)
     ap.add_argument("--out", default=PARAMS_FILE)
     ap.add_argument("--summary", default=None, help="JSON summary of the data and the training")
+    ap.add_argument("--data", default=None,
+                    help="train on a labelled set saved by an earlier run (its *_data.npz) instead of labelling; "
+                         "the summary's system list is kept from the JSON beside it")
     args = ap.parse_args(argv)
 
     def log(msg):
         print(msg, flush=True)
 
     t0 = time.time()
+    if args.data:
+        z = np.load(args.data)
+        F, y, t, m = z["F"], z["y"], z["t"], z["m"]
+        prev = json.load(open(args.data.replace("_data.npz", ".json")))
+        log(f"dataset {args.data}: {F.shape[0]} steps, {int(y.sum())} labelled FP8 (labelled earlier: "
+            f"{prev.get('solves')} solves)")
+        params, losses, acc = train(F, y, t, m, iters=args.iters, log=log)
+        os.makedirs(os.path.dirname(args.out), exist_ok=True)
+        params.astype("<f4").tofile(args.out)
+        log(f"trained: accuracy {acc:.3f} on the training steps; parameters -> {args.out}")
+        if args.summary:
+            prev.update(losses=losses, train_accuracy=acc, iters=args.iters, train_seconds=time.time() - t0)
+            with open(args.summary, "w") as f:
+                json.dump(prev, f, indent=1)
+        return
     rows = []
     summ = []
     solvers = {}
     nsolves = 0
-    for kind, n, seed, kappa in training_systems(args.systems):
+    sizes = tuple(int(v) for v in args.sizes.split(","))
+    skipped = 0
+    for kind, n, seed, kappa in training_systems(args.systems, sizes=sizes):
+        if sum(s.solves for s in solvers.values()) >= args.solves:
+            break
         if n not in solvers:
             solvers[n] = Solver(n, args.b)
         sysm = System(kind, n, seed, kappa)
         lab = label_system(solvers[n], sysm, args.slack, log)
         del sysm
         if lab is None:
+            skipped += 1
             continue
         F, y, t, m, info = lab
         rows.append((F, y, t, m))
         summ.append(info)
+    nsolves = sum(s.solves for s in solvers.values())
     for s in solvers.values():
         s.close()
     F = np.concatenate([r[0] for r in rows])
     y = np.concatenate([r[1] for r in rows])
     t = np.concatenate([r[2] for r in rows])
     m = np.concatenate([r[3] for r in rows])
-    log(f"dataset: {F.shape[0]} steps from {len(rows)} systems, {int(y.sum())} labelled FP8 "
-        f"({time.time() - t0:.0f} s)")
+    log(f"dataset: {F.shape[0]} steps from {len(rows)} systems ({skipped} skipped: all-FP16 did not converge), "
+        f"{nsolves} solves, {int(y.sum())} labelled FP8 ({time.time() - t0:.0f} s)")
     params, losses, acc = train(F, y, t, m, iters=args.iters, log=log)
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     params.astype("<f4").tofile(args.out)
@@ -323,7 +357,8 @@ def main(argv=None):
     if args.summary:
         with open(args.summary, "w") as f:
             json.dump(dict(systems=summ, samples=int(F.shape[0]), fp8_labels=int(y.sum()), losses=losses,
-                           train_accuracy=acc, b=args.b, slack=args.slack, iters=args.iters,
+                           train_accuracy=acc, b=args.b, slack=args.slack, iters=args.iters, sizes=list(sizes),
+                           solves=nsolves, skipped_systems=skipped, nrefl=8,
                            seconds=time.time() - t0), f, indent=1)
         np.savez(args.summary.replace(".json", "_data.npz"), F=F, y=y, t=t, m=m)
 

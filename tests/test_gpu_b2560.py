@@ -73,20 +73,26 @@ def _factor_vs_oracle(orc, n, alt, group4, sched):
     tol = 2.0**-8 if sched is None or not any(sched) else 2.0**-6
     assert dl <= tol * lmax
     assert du <= tol
-    # backward error of the GPU factors on 64 sampled rows, against the
-    # precision of the Schur panels (FP16: 2^-11; FP8 e4m3: 2^-4)
+    # backward error |A - L U| / (|L||U|) on 64 sampled rows: the GPU
+    # factors against the oracle's (the reference's method has this error
+    # too: the -alt products multiply FP16-rounded panels by FP16-rounded
+    # inverses of the diagonal blocks, so it grows with their condition)
     rng = np.random.default_rng(n)
     rows = np.sort(rng.choice(n, 64, replace=False))
-    G64 = got.astype(np.float64)
-    Lr = np.tril(G64[rows], -1)
-    Lr[np.arange(64), rows] = 1.0
-    Uf = np.triu(G64)
-    R = Lr @ Uf - Ao[rows].astype(np.float64)
-    S = np.abs(Lr) @ np.abs(Uf)
-    u = 2.0**-9 if sched is None or not any(sched) else 2.0**-3
-    ratio = float(np.max(np.abs(R) / (u * S)))
-    print(f"  sampled backward error / (u |L||U|) = {ratio:.3e}", flush=True)
-    assert ratio < 1.0
+
+    def backward(F):
+        F64 = F.astype(np.float64)
+        # row r of the unit lower factor: columns j < r, 1 at j = r
+        Lr = np.where(np.arange(n)[None, :] < rows[:, None], F64[rows], 0.0)
+        Lr[np.arange(64), rows] = 1.0
+        Uf = np.triu(F64)
+        R = Lr @ Uf - Ao[rows].astype(np.float64)
+        S = np.abs(Lr) @ np.abs(Uf)
+        return float(np.max(np.abs(R) / S))
+
+    bg, bo = backward(got), backward(want)
+    print(f"  sampled backward error / (|L||U|): GPU {bg:.3e}, oracle {bo:.3e}", flush=True)
+    assert bg <= 1.5 * bo + 2.0**-20
     return tim
 
 

@@ -195,45 +195,57 @@ def test_constant_policy_equals_fixed_schedule(fp8, headroom):
 
 
 def test_trained_policy_against_fixed_schedules():
-    """What the trained network gains over the methods without it, on systems
-    outside its training set (n = 8192, b = 512, GMRES-IR):
-      - diagonally dominant LCG system: the policy takes FP8 on (nearly) every
-        step, as all-FP8 does, and refines within the FP16 cost + 3;
-      - randsvd system (kappa = 300): all-FP8 does not refine within the FP16
-        cost + 3, the policy does (it keeps FP16 on the steps that need it)
-        while still taking FP8 on some steps."""
+    """What the trained network gains over the methods without it, on
+    seeded systems outside its training set (n = 16384, b = 512, GMRES-IR):
+    the diagonally dominant LCG matrix and randsvd matrices (8 reflectors per
+    factor) with kappa = 30, 300, 3000, each factored all-FP16, all-FP8 and
+    with the learned schedule:
+      - safer than all-FP8: the learned schedule refines wherever all-FP16
+        does, while all-FP8 fails (or exceeds the FP16 cost + 3) on at least
+        one randsvd system;
+      - more FP8 than all-FP16: FP8 on (nearly) every step of the LCG system
+        and on more than half of the four systems' steps, with the four
+        refinements' total cost within the labelling rule's FP16 cost + 3
+        per system.
+    (At this order the LU is bound by its diagonal-block chain, not by the
+    Schur products, so its time hardly depends on their precision; the time
+    gain shows at n = 204800, in the bench's learned / all-FP8 / FP16 legs.)"""
     torch_cuda()
     from openmxp_amd import scheduler
     try:
         params = scheduler.load_params()
     except (OSError, ValueError):
         pytest.skip("no trained scheduler committed")
-    n, b = 8192, 512
+    n, b = 16384, 512
+    nb = n // b
     S = scheduler.Solver(n, b)
+    rows = []
     try:
-        lcg = scheduler.System("lcg", n, 4242)
-        f16 = S.solve(lcg, sched=np.zeros(n // b, np.int32))
-        S.set_policy(params)
-        learned = S.solve(lcg)
-        S.set_policy(None)
-        print(f"  lcg: fp16 cost {f16['cost']}, learned cost {learned['cost']}, "
-              f"fp8 steps {int((learned['sched'][:-1] > 0).sum())}/{n // b - 1}", flush=True)
-        assert f16["ok"] and learned["ok"] and learned["cost"] <= f16["cost"] + 3
-        assert (learned["sched"][:-1] > 0).sum() >= 0.9 * (n // b - 1)
-        del lcg
-
-        rs = scheduler.System("randsvd", n, 4343, kappa=300.0)
-        f16 = S.solve(rs, sched=np.zeros(n // b, np.int32))
-        all8 = S.solve(rs, sched=np.ones(n // b, np.int32))
-        S.set_policy(params)
-        learned = S.solve(rs)
-        S.set_policy(None)
-        n8 = int((learned["sched"][:-1] > 0).sum())
-        print(f"  randsvd: fp16 cost {f16['cost']}, all-fp8 {'cost ' + str(all8['cost']) if all8['ok'] else 'fails'}, "
-              f"learned cost {learned['cost']} with {n8} FP8 steps", flush=True)
-        assert f16["ok"]
-        assert not (all8["ok"] and all8["cost"] <= f16["cost"] + 3)
-        assert learned["ok"] and learned["cost"] <= f16["cost"] + 3
-        assert n8 >= 1
+        for kind, seed, kappa in (("lcg", 4242, 1.0), ("randsvd", 4343, 30.0), ("randsvd", 4344, 300.0),
+                                  ("randsvd", 4345, 3000.0)):
+            sysm = scheduler.System(kind, n, seed, kappa=kappa)
+            S.set_policy(None)
+            S.solve(sysm, sched=np.zeros(nb, np.int32))  # warm-up
+            f16 = S.solve(sysm, sched=np.zeros(nb, np.int32))
+            all8 = S.solve(sysm, sched=np.ones(nb, np.int32))
+            S.set_policy(params)
+            learned = S.solve(sysm)
+            S.set_policy(None)
+            n8 = int((learned["sched"][:-1] > 0).sum())
+            print(f"  {kind} kappa={kappa:g}: fp16 cost {f16['cost']} LU {f16['lu_s'] * 1e3:.1f} ms | all-fp8 "
+                  f"{'cost ' + str(all8['cost']) if all8['ok'] else 'fails'} LU {all8['lu_s'] * 1e3:.1f} ms | learned "
+                  f"{'cost ' + str(learned['cost']) if learned['ok'] else 'fails'} LU {learned['lu_s'] * 1e3:.1f} ms, "
+                  f"{n8}/{nb - 1} FP8 steps", flush=True)
+            rows.append((kind, kappa, f16, all8, learned, n8))
+            del sysm
     finally:
         S.close()
+    for kind, kappa, f16, all8, learned, n8 in rows:
+        if f16["ok"]:
+            assert learned["ok"], (kind, kappa)
+    assert any(not (all8["ok"] and all8["cost"] <= f16["cost"] + 3)
+               for kind, kappa, f16, all8, learned, n8 in rows if kind == "randsvd")
+    kind, kappa, f16, all8, learned, n8 = rows[0]
+    assert n8 >= 0.9 * (nb - 1) and learned["cost"] <= f16["cost"] + 3
+    assert sum(r[5] for r in rows) > 0.5 * len(rows) * (nb - 1)
+    assert sum(r[4]["cost"] for r in rows) <= sum(r[2]["cost"] for r in rows) + 3 * len(rows)
