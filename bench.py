@@ -75,11 +75,16 @@ def parse():
     ap.add_argument("--no-ir-kernels", action="store_true", help="skip the triangular-solve / residual timings")
     ap.add_argument("--randsvd-kappas", default="1e2,1e4,1e6",
                     help="condition numbers of the randsvd legs (comma separated; empty: none)")
+    ap.add_argument("--no-randsvd-fp32", action="store_true",
+                    help="skip the FP32-class solves of the randsvd legs beyond FP16 (kappa * 2^-11 >= 1)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (the reference kernel for PMC byte counters)")
     ap.add_argument("--gemm-sweep", action="store_true",
                     help="run only the trailing-update GEMMs of one factorisation, FP16 in pairs then FP8 in groups of four "
                          "on the default stream, for counter collection")
+    ap.add_argument("--summarize-trace", nargs=2, metavar=("TRACE_CSV", "LEGS"),
+                    help="print per-kernel totals and per-method trailing-update rates from a rocprofv3 "
+                         "--kernel-trace CSV of a bench run (LEGS: the solves in order, leg:group,...), and exit")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the Schur GEMM traffic from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
                          "counter_collection files of a --copy-calibration run, and exit")
@@ -209,12 +214,14 @@ def agg(res, flops):
                 steps_total=last["steps_total"], fell_back=any(r["fell_back"] for r in res))
 
 
-def randsvd_table(n, b, alt, kappas, policy, seed, group=4, nrefl=8):
+def randsvd_table(n, b, alt, kappas, policy, seed, group=4, nrefl=8, fp32=True):
     """The conditioned systems: randsvd A = U diag(sigma) V^T (sigma
     geometric from 1 to 1/kappa, U and V products of nrefl seeded Householder
     reflectors) for each kappa, solved with the Schur update in FP16 on every
     step, FP8 on every step, and the learned schedule (FP16 fallback when its
-    refinement fails), all with groups of `group` panels and GMRES-IR.  The
+    refinement fails), all with groups of `group` panels and GMRES-IR; where
+    kappa * 2^-11 >= 1 (beyond FP16) also with every step in the FP32 class
+    (schedule -1: FP32 trailing products on the panels as they stand).  The
     forward error is against the same solution refined on with three more
     GMRES-IR corrections to the FP64 floor (residuals from the FP64 low-rank
     description)."""
@@ -227,8 +234,10 @@ def randsvd_table(n, b, alt, kappas, policy, seed, group=4, nrefl=8):
     for kappa in kappas:
         sysm = scheduler.System("randsvd", n, seed, kappa=kappa, nrefl=nrefl)
         x = torch.empty(n, dtype=torch.float64, device="cuda")
-        for meth in ("fp16", "fp8", "learned"):
+        for meth in ("fp16", "fp8", "learned", "fp32"):
             if meth == "learned" and policy is None:
+                continue
+            if meth == "fp32" and (not fp32 or kappa * 2.0**-11 < 1.0):
                 continue
             h = C.c_void_p()
             flags = _lib.SOLVER_FP8 | (_lib.SOLVER_GROUP4 if group == 4 else 0)
@@ -251,7 +260,7 @@ def randsvd_table(n, b, alt, kappas, policy, seed, group=4, nrefl=8):
                                            1, 50, 50, 1e-10, 16.0, 0, out.ctypes.data, None)
                 return info, rc, out, (sa if sched is not None else None)
 
-            sched = {"fp16": [0] * nb, "fp8": [1] * nb, "learned": None}[meth]
+            sched = {"fp16": [0] * nb, "fp8": [1] * nb, "learned": None, "fp32": [-1] * nb}[meth]
             sysm.generate()
             t0 = time.perf_counter()
             info, rc, out, _ = lu_ir(sched)
@@ -450,6 +459,67 @@ def summarize_pmc(fetch_csv, write_csv, n, b):
         print(f"{tag}.traffic_bytes_per_launch: {(sum(fetch) + sum(write)) / max(nl, 1):.6e}")
 
 
+def summarize_trace(trace_csv, n, b, legs):
+    """Per-kernel totals and the trailing-update GEMM rates per method from a
+    rocprofv3 --kernel-trace CSV of a bench run.  The trace is split into the
+    bench's solves at each matrix generation (one k_matgen_block dispatch
+    per solve); `legs` names them in order with their group size, e.g.
+    "plain:2,plain:2,g4:4,g4:4,learned:4,learned:4,fp8:4,fp8:4".  A
+    dispatch of the beta = 1 GEMM is a trailing update when its grid is the
+    square tile grid of m = n - G (g + 2) b for the solve's G (look-ahead
+    products are m x b)."""
+    import csv
+    from collections import defaultdict
+    rows = list(csv.DictReader(open(trace_csv)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    plan = [(l.split(":")[0], int(l.split(":")[1])) for l in legs.split(",")]
+    tot = defaultdict(lambda: [0, 0.0])
+    rate = defaultdict(lambda: [0, 0.0, 0.0])  # launches, flops, seconds
+    seg = -1
+    for r in rows:
+        name = r["Kernel_Name"]
+        dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
+        bare = name.replace("void ", "").replace("(anonymous namespace)::", "")
+        key = bare.split("(")[0]
+        if key.startswith("_Z"):  # unmangled names only
+            key = bare[:60]
+        tot[key][0] += 1
+        tot[key][1] += dur
+        if "k_matgen_block" in name:
+            seg += 1
+            continue
+        if seg < 0 or seg >= len(plan):
+            continue
+        leg, G = plan[seg]
+        beta1 = ("k_gemm_f16<false" in name or "k_gemm_f8<false" in name or "k_gemm_half<false, false" in name
+                 or "k_gemm_half<true, false" in name)
+        if not beta1:
+            continue
+        wg = int(r["Grid_Size_X"]) // int(r["Workgroup_Size_X"])
+        half = "k_gemm_half" in name
+        for g in range(n // (G * b)):
+            m = n - G * (g + 2) * b
+            if m <= 0:
+                break
+            tiles = (m // 256) * (m // (128 if half else 256))
+            if wg == tiles:
+                prec = "fp8" if ("k_gemm_f8" in name or "k_gemm_half<true" in name) else "fp16"
+                k = rate[(leg, prec)]
+                k[0] += 1
+                k[1] += 2.0 * m * m * G * b
+                k[2] += dur
+                break
+    grand = sum(v[1] for v in tot.values())
+    print(f"workload: bench solves at n={n} b={b}, in order: {legs}")
+    print("trailing-update GEMM per method (per-dispatch durations from the trace):")
+    for (leg, prec), (nl, fl, sec) in sorted(rate.items()):
+        print(f"  {leg:8s} {prec}: {nl} launches, {fl:.4e} flops, {sec * 1e3:.1f} ms, {fl / sec * 1e-12:.1f} TFLOP/s")
+    print("| kernel | calls | total ms | avg us | share |")
+    print("|---|---:|---:|---:|---:|")
+    for key, (c, sec) in sorted(tot.items(), key=lambda kv: -kv[1][1])[:24]:
+        print(f"| `{key}` | {c} | {sec * 1e3:.1f} | {sec / c * 1e6:.1f} | {100.0 * sec / grand:.1f}% |")
+
+
 def bulk_shapes(n, b, G=2):
     """Row counts m of the trailing updates of one factorisation with panels
     in groups of G: group g updates A[f:, f:] with f = G (g + 2) b, depth
@@ -530,6 +600,9 @@ def main():
             log("copied 1 GiB device to device (PMC calibration)")
         gemm_sweep(args.n, args.b, args.seed)
         return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace[0], args.n, args.b, args.summarize_trace[1])
+        return
     if args.summarize_pmc:
         summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.n, args.b)
         return
@@ -588,7 +661,8 @@ def main():
     rsvd = None
     kappas = [float(v) for v in args.randsvd_kappas.split(",") if v.strip()]
     if kappas:
-        rsvd = randsvd_table(n, b, args.alt, kappas, policy, args.seed, group=args.group)
+        rsvd = randsvd_table(n, b, args.alt, kappas, policy, args.seed, group=args.group,
+                             fp32=not args.no_randsvd_fp32)
     achieved = a16["gemm_flops"] / (a16["gemm_ms"] * 1e-3) * 1e-12
     cpu = gpu_small = None
     if not args.no_cpu:
@@ -638,7 +712,7 @@ def main():
         # the schedule chose FP8; all of them when fp8_steps == steps)
         line["roofline"] = {"bound": "mfma", "achieved": round(achL, 1), "peak": PEAK_FP8_TFLOPS, "unit": "TFLOP/s",
                             "frac": round(achL / PEAK_FP8_TFLOPS, 4), "traffic": pmc_traffic("f8", n, b, args.group),
-                            "kernel": f"FP8 Schur GEMM (k_gemm_half, 256 x 128 tiles) trailing update per {args.group} panels (depth {args.group}b, "
+                            "kernel": f"FP8 Schur GEMM (omx_gemm_f8_packed_window) trailing update per {args.group} panels (depth {args.group}b, "
                                       "alpha=-1, beta=1), learned run",
                             "launches": aL["gemm_launches"],
                             "fp8_steps": aL["fp8_steps"], "steps": aL["steps_total"]}

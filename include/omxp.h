@@ -513,7 +513,11 @@ int omx_solver_reserve_gmres(omx_solver *s, int restart);
  * of step k overlapped with the trailing update of step k on a second
  * stream.  sched (host, n/b ints, optional) fixes each step's Schur panels:
  * 0 FP16, 1 + h scaled FP8 e4m3 with tile-scale headroom h (0..3; FP8
- * requires OMX_SOLVER_FP8).  sched = NULL: the learned policy if one was set
+ * requires OMX_SOLVER_FP8), -1 FP32 (the step's panels by FP32 TRSM and its
+ * Schur products on the FP32 matrix cores straight from A: the precision
+ * class for systems whose condition number times the FP16 unit roundoff is
+ * not small; no counterpart in the reference, whose Schur update is always
+ * FP16).  sched = NULL: the learned policy if one was set
  * (omx_solver_set_policy), otherwise FP16 throughout.  Blocks until done.
  * timings (host, optional, 5 doubles): wall ms, summed ms of the
  * trailing-update GEMMs (HIP events on their stream), their count, their

@@ -246,6 +246,24 @@ k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char 
             yd[q] = *reinterpret_cast<const half8 *>(ly + ((wn * 4 + q) * 2 + kk) * 1024 + lane * 16);
         }
     };
+    // copy i (0..7) of stage st: X (i even) or Y (i odd) segment wave*4 + i/2
+    auto issue_one = [&](int st, int i) {
+        const int64_t base = (int64_t)(st >> 1) * kChunk + (st & 1) * 2048;
+        const int seg = wave * 4 + (i >> 1);
+        const int64_t off = base + (seg >> 1) * 4096 + (seg & 1) * 1024 + lane * 16;
+        if (i & 1)
+            gl_lds16(ys + off, smem[st % kNbuf][1] + seg * 1024);
+        else
+            gl_lds16(xs + off, smem[st % kNbuf][0] + seg * 1024);
+    };
+    // fragment read i (0..7): X row i (i < 4) or Y row i - 4
+    auto frag_one = [&](half8(&xd)[4], half8(&yd)[4], int st, int kk, int i) {
+        if (i < 4)
+            xd[i] = *reinterpret_cast<const half8 *>(smem[st % kNbuf][0] + ((wm * 4 + i) * 2 + kk) * 1024 + lane * 16);
+        else
+            yd[i - 4] =
+                *reinterpret_cast<const half8 *>(smem[st % kNbuf][1] + ((wn * 4 + i - 4) * 2 + kk) * 1024 + lane * 16);
+    };
 
     floatx16 acc[4][4];
 #pragma unroll
@@ -291,9 +309,25 @@ k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char 
         }
         __builtin_amdgcn_sched_barrier(0);
         if (kSteady) {
-            issue(st + kNbuf - 1);
-            frags(xa, ya, st + 1, 0);
-        } else if (more) {
+            // 8 groups of 2 MFMAs, one ring-slot copy and one fragment read,
+            // in this order (fenced: left to itself the scheduler issues the
+            // 8 copies back to back, each behind its M0 write, while the
+            // matrix pipe idles)
+            const int sn = st + kNbuf - 1;
+#pragma unroll
+            for (int g = 0; g < 8; ++g) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int mi = (2 * g + h) >> 2, ni = (2 * g + h) & 3;
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(yb[ni], xb[mi], acc[mi][ni], 0, 0, 0);
+                }
+                issue_one(sn, g);
+                frag_one(xa, ya, st + 1, 0, g);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            return;
+        }
+        if (more) {
             if (st + kNbuf - 1 < nks) issue(st + kNbuf - 1);
             frags(xa, ya, st + 1, 0);
         }
@@ -302,14 +336,6 @@ k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char 
 #pragma unroll
             for (int ni = 0; ni < 4; ++ni)
                 acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(yb[ni], xb[mi], acc[mi][ni], 0, 0, 0);
-        if (kSteady) {
-#pragma unroll
-            for (int g = 0; g < 8; ++g) {
-                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // 2 MFMA
-                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 DS read
-                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 VMEM load
-            }
-        }
         __builtin_amdgcn_sched_barrier(0);
     };
     int st = 0;
@@ -371,6 +397,28 @@ k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *
             yd[q] = intx8{ylo[0], ylo[1], ylo[2], ylo[3], yhi[0], yhi[1], yhi[2], yhi[3]};
         }
     };
+    // copy i (0..7) of stage st: X (i even) or Y (i odd) segment wave*4 + i/2
+    auto issue_one = [&](int st, int i) {
+        const int64_t base = (int64_t)(st >> 1) * kChunk + (st & 1) * 2048;
+        const int seg = wave * 4 + (i >> 1);
+        const int64_t off = base + (seg >> 1) * 4096 + (seg & 1) * 1024 + lane * 16;
+        if (i & 1)
+            gl_lds16(ys + off, smem[st % kNbuf][1] + seg * 1024);
+        else
+            gl_lds16(xs + off, smem[st % kNbuf][0] + seg * 1024);
+    };
+    // fragment i (0..7): X row i (i < 4) or Y row i - 4, both halves
+    auto frag_one = [&](intx8(&xd)[4], intx8(&yd)[4], int st, int i) {
+        const char *l = (i < 4 ? smem[st % kNbuf][0] + (wm * 4 + i) * 2048
+                               : smem[st % kNbuf][1] + (wn * 4 + i - 4) * 2048) + lane * 16;
+        const intx4 lo = *reinterpret_cast<const intx4 *>(l);
+        const intx4 hi = *reinterpret_cast<const intx4 *>(l + 1024);
+        const intx8 v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        if (i < 4)
+            xd[i] = v;
+        else
+            yd[i - 4] = v;
+    };
 
     floatx16 acc[4][4];
 #pragma unroll
@@ -410,9 +458,21 @@ k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *
         }
         __builtin_amdgcn_sched_barrier(0);
         if (kSteady) {
-            issue(cs + kNbuf - 1);
-            frags(xn, yn, cs + 1);
-        } else if (more) {
+            // 8 groups of one MFMA, one ring-slot copy and one fragment (two
+            // reads), in this order (fenced, as in the FP16 kernel)
+            const int sn = cs + kNbuf - 1;
+#pragma unroll
+            for (int g = 0; g < 8; ++g) {
+                const int mi = 2 + (g >> 2), ni = g & 3;
+                acc[mi][ni] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(yc[ni], xc[mi], acc[mi][ni], 0, 0, 0,
+                                                                              sy, 0, sx);
+                issue_one(sn, g);
+                frag_one(xn, yn, cs + 1, g);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            return;
+        }
+        if (more) {
             if (cs + kNbuf - 1 < nks) issue(cs + kNbuf - 1);
             frags(xn, yn, cs + 1);
         }
@@ -422,14 +482,6 @@ k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *
             for (int ni = 0; ni < 4; ++ni)
                 acc[mi][ni] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(yc[ni], xc[mi], acc[mi][ni], 0, 0, 0,
                                                                               sy, 0, sx);
-        if (kSteady) {
-#pragma unroll
-            for (int g = 0; g < 8; ++g) {
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
-                __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // 2 DS reads
-                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 VMEM load
-            }
-        }
         __builtin_amdgcn_sched_barrier(0);
     };
     // nks is a multiple of 4 (k % 256 == 0): stages come in pairs
@@ -649,6 +701,13 @@ k_gemm_half(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char
             for (int t = 0; t < 3; ++t) gl_lds16(src[t] + off, l + (3 * wave + t) * 1024);
         }
     };
+    // copy i (0 .. 3 kP - 1) of k-step `step`: slot kP step + i / 3, block
+    // 3 wave + i % 3
+    auto issue_one = [&](int step, int i) {
+        const int s = kP * step + i / 3, t = i % 3;
+        const int64_t off = (int64_t)(s >> 2) * kChunk + (s & 3) * 1024;
+        gl_lds16(src[t] + off, smem + (s % kSlots) * kSlotBytes + (3 * wave + t) * 1024);
+    };
     // A lane's operand of one k-step for fragment row f (0-7 X, 8-11 Y): 16
     // bytes from each of the step's slots.  FP16: one 16-column MFMA operand
     // per slot; FP8: the 32 bytes of one 64-column MFMA operand.
@@ -702,7 +761,7 @@ k_gemm_half(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char
         if (more) {
             wait_steps<kLoads>(Cfg::kWait);
             stage_barrier();
-            if (Cfg::kIssue) issue(st + kR);
+            if (!kDB && Cfg::kIssue) issue(st + kR);
         }
         int sx = 0, sy = 0;
         if constexpr (kFp8) {
@@ -711,28 +770,21 @@ k_gemm_half(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char
             sy = 127 - ye[kt];
         }
         if constexpr (kDB) {
-            if (more) {
+            // the step's 8 (mi, ni) products in groups, each followed by one
+            // of the refill copies and one of the next step's fragments
+            // (fenced: left to itself the scheduler issues the copies back to
+            // back, each behind its M0 write, while the matrix pipe idles)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) xn[q] = frag(st + 1, wm * 4 + q);
-#pragma unroll
-                for (int q = 0; q < 2; ++q) yn[q] = frag(st + 1, 8 + wn * 2 + q);
-            }
-#pragma unroll
-            for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-                for (int ni = 0; ni < 2; ++ni) mfma(acc[mi][ni], yc[ni], xc[mi], sy, sx);
-            if (more && Cfg::kIssue) {
-                // MFMA instructions, LDS reads and copies of the step spread
-                // evenly: 8 (FP8, FP16 depth 16) or 16 MFMAs, 6 or 12 reads,
-                // 3 or 6 copies
-                constexpr int kM = (kFp8 || kP == 1) ? 8 : 16, kD = 6 * kP, kV = 3 * kP;
-                constexpr int kG = kV;
-#pragma unroll
-                for (int g = 0; g < kG; ++g) {
-                    __builtin_amdgcn_sched_group_barrier(0x008, kM / kG, 0);  // MFMA
-                    __builtin_amdgcn_sched_group_barrier(0x100, kD / kG, 0);  // DS read
-                    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);        // VMEM
+            for (int g = 0; g < 8; ++g) {
+                mfma(acc[g >> 1][g & 1], yc[g & 1], xc[g >> 1], sy, sx);
+                if (Cfg::kIssue && g < 3 * kP) issue_one(st + kR, g);
+                if (more && g < 6) {
+                    if (g < 4)
+                        xn[g] = frag(st + 1, wm * 4 + g);
+                    else
+                        yn[g - 4] = frag(st + 1, 8 + wn * 2 + g - 4);
                 }
+                __builtin_amdgcn_sched_barrier(0);
             }
         } else {
             // the next step's X fragment of row mi is read once the current
@@ -785,11 +837,15 @@ int gemm_args(int64_t m, int64_t n, int64_t k, int64_t kstage, const void *X, in
     return OMX_OK;
 }
 
-// The trailing-update kernel family: 256 x 128 tiles two workgroups per CU
-// (default) or 256 x 256 tiles one per CU (OMX_GEMM_TILE=256).
+// The trailing-update kernel family: 256 x 256 tiles one workgroup per CU
+// (default) or 256 x 128 tiles two per CU (OMX_GEMM_TILE=half).  The second
+// is faster on a GPU that runs nothing else (FP8 2.4-2.5 against 2.35-2.4
+// PFLOP/s at k = 10240), the first inside the LU, where the critical
+// stream's kernels run beside the trailing update (FP8 1.94 against 1.60,
+// FP16 1.02 against 0.87 PFLOP/s).
 bool use_half_tiles() {
     const char *e = getenv("OMX_GEMM_TILE");
-    return !(e && !strcmp(e, "256"));
+    return e && !strcmp(e, "half");
 }
 
 // the K window [k0, k0 + k) of a packed panel of ldk columns

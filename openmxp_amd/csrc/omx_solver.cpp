@@ -444,16 +444,27 @@ static int schur_window(omx_solver *s, int64_t i, int64_t k, int64_t k0, int64_t
 // A(r0.., c0..) (m x nn) -= the panels i0 .. i1 (all in one group), in
 // order; each run of consecutive panels packed alike is one product over
 // its K windows (scaled FP16 panels, whose scales are per panel, one by one).
+static int prec_class(int p) { return p > 0 ? 1 : (p < 0 ? -1 : 0); }
+
 static int apply_panels(omx_solver *s, int64_t i0, int64_t i1, int64_t r0, int64_t c0, int64_t m, int64_t nn,
                         float *A, int64_t lda, hipStream_t st) {
     if (m <= 0 || nn <= 0) return OMX_OK;
     const int64_t b = s->b, G = s->G;
     for (int64_t i = i0; i <= i1;) {
         int64_t j = i;
-        const bool f8 = s->prec[i] > 0;
-        if (f8 || !s->scale16)
-            while (j + 1 <= i1 && (j + 1) / G == i / G && (s->prec[j + 1] > 0) == f8) ++j;
-        RC(schur_window(s, i, (j - i + 1) * b, pwin(i, b, G), r0, c0, m, nn, A, lda, st));
+        const int cls = prec_class(s->prec[i]);
+        if (cls != 0 || !s->scale16)
+            while (j + 1 <= i1 && (j + 1) / G == i / G && prec_class(s->prec[j + 1]) == cls) ++j;
+        if (cls < 0) {
+            // FP32 class: C -= A(r0.., ib..jb+b) A(ib..jb+b, c0..), the
+            // panels as factored (column panels side by side, row panels
+            // stacked), on the FP32 matrix cores
+            const int64_t ib = i * b, k = (j - i + 1) * b;
+            RC(omx_sgemm_f32(m, nn, k, -1.0f, A + r0 + ib * lda, lda, A + ib + c0 * lda, lda, 1.0f,
+                             A + r0 + c0 * lda, lda, st));
+        } else {
+            RC(schur_window(s, i, (j - i + 1) * b, pwin(i, b, G), r0, c0, m, nn, A, lda, st));
+        }
         i = j + 1;
     }
     return OMX_OK;
@@ -477,6 +488,7 @@ static int update_block(omx_solver *s, int64_t t, int64_t i0, int64_t i1, float 
 // buffer window, in the precision s->prec[i].
 static int pack_panel(omx_solver *s, int64_t i, int have_max, int64_t rest, const float *Acol, const float *Arow,
                       int64_t lda, hipStream_t st) {
+    if (s->prec[i] < 0) return OMX_OK;  // FP32 class: the products read the panels in A
     const int64_t b = s->b, G = s->G, ldk = G * b;
     const int buf = pbuf(i, G);
     const int64_t rstrip = pwin(i, b, G) / OMX_STRIP;  // packed rows start at (i % G) b
@@ -595,7 +607,7 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
     hipStream_t cs = s->crit, bs = s->bulk;
     if (sched)
         for (int64_t k = 0; k < nb; ++k)
-            if (sched[k] < 0 || sched[k] > 4 || (sched[k] && !s->fp8)) return OMX_ERR_ARG;
+            if (sched[k] < -1 || sched[k] > 4 || (sched[k] > 0 && !s->fp8)) return OMX_ERR_ARG;
     // sched given: fixed choices; else the policy if one is set; else FP16
     const bool learned = !sched && s->has_policy;
     const bool stats = s->fp8 != 0;
@@ -632,7 +644,10 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
             const double rb2 = (double)rest * b * b;
             if (s->skip & OMX_SKIP_TRSM) {
                 // panels left as they are (the reference's skip bit 2)
-            } else if (i == 0 || s->alt == 0) {
+            } else if (i == 0 || s->alt == 0 || (sched && sched[i] < 0)) {
+                // FP32 panels by TRSM: step 0 (the reference's), -alt 0, and
+                // the FP32 Schur class (whose products take the panels as
+                // they are, so the FP16 inverse products would limit them)
                 TIMED(T, kT_TRSM_L, cs, rb2, omx_trsm_f32(OMX_TRSM_RUNN, rest, b, Akk, lda, Acol, lda, cs));
                 TIMED(T, kT_TRSM_R, cs, rb2, omx_trsm_f32(OMX_TRSM_LLNU, b, rest, Akk, lda, Arow, lda, cs));
             } else {

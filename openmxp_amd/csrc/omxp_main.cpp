@@ -1,11 +1,12 @@
 // omxp_main.cpp -- command-line driver with the reference's surface
 // (main.cpp:85-252, :273-736): positional N B [P Q] and the flags -solv,
-// -alt, -log, -progress, -init, plus -prec (fp16 | fp8 | learned), -policy,
+// -alt, -log, -progress, -init, plus -prec (fp16 | fp8 | learned | fp32), -policy,
 // -ir, -seed, -maxit, -restart, -device, --highammat (the Fasi-Higham
 // matrix family, main.cpp:239-240, highammgen.hpp:4-30) and -randsvd kappa
 // (a conditioned matrix of 2-norm condition number kappa).  One GPU, 1x1
 // grid.  With -prec learned the MLP policy picks each step's Schur
-// precision; whenever FP8 steps were used (-prec fp8 or learned) and the
+// precision; -prec fp32 runs every trailing update in FP32 (panels read
+// straight from A, for systems whose kappa * 2^-11 exceeds 1); whenever FP8 steps were used (-prec fp8 or learned) and the
 // refinement then fails, the system is generated again and factored in FP16
 // (inside the timed region).  -skip is the reference's bit-mask (IR, GETRF,
 // TRSM, GEMM; main.cpp:68-70); -progress k prints the reference's
@@ -53,7 +54,7 @@ struct Opts {
 void usage() {
     std::printf(
         "usage: omxp N B [P Q] [-solv 0|1] [-alt 0|1|2] [-log 0|1] [-progress k] [-init 1]\n"
-        "            [-prec fp16|fp8|learned] [-policy file] [-ir classic|gmres] [-seed s] [-maxit k]\n"
+        "            [-prec fp16|fp8|learned|fp32] [-policy file] [-ir classic|gmres] [-seed s] [-maxit k]\n"
         "            [-restart m] [-device d] [-group 2|4] [-timer]\n"
         "            [--highammat | --highammat-paper] [-cond kappa] [-rho r]\n"
         "            [-randsvd kappa] [-nrefl k] [-skip mask]\n");
@@ -152,8 +153,8 @@ int main(int argc, char **argv) {
         if (nf == 0) nf = 1;
         o.n = nf * o.b;
     }
-    if (o.prec != "fp16" && o.prec != "fp8" && o.prec != "learned") {
-        std::fprintf(stderr, "-prec must be fp16, fp8 or learned\n");
+    if (o.prec != "fp16" && o.prec != "fp8" && o.prec != "learned" && o.prec != "fp32") {
+        std::fprintf(stderr, "-prec must be fp16, fp8, learned or fp32\n");
         return 2;
     }
     if (o.group != 0 && o.group != 2 && o.group != 4) {
@@ -162,6 +163,7 @@ int main(int argc, char **argv) {
     }
     const bool learned = (o.prec == "learned");
     const bool fp8 = (o.prec == "fp8") || learned;
+    const bool fp32 = (o.prec == "fp32");
     // the Higham matrix in the library's signed form: a_ij = -beta + ab i
     // above, 1 + ab i on, -alpha + ab j below the diagonal
     double h_alpha = 0, h_beta = 0;
@@ -198,7 +200,7 @@ int main(int argc, char **argv) {
     std::printf("===============================================================================\n");
     std::printf("\t%s\n", omx_version());
     std::printf("\tn=%ld b=%ld 1x1 grid, alt=%d, Schur precision %s, IR %s (maxit %d)\n", n, b, o.alt,
-                learned ? "learned FP8/FP16 per step" : (fp8 ? "FP8 e4m3 (pow2 tile scale)" : "FP16"),
+                learned ? "learned FP8/FP16 per step" : (fp8 ? "FP8 e4m3 (pow2 tile scale)" : (fp32 ? "FP32" : "FP16")),
                 o.ir ? "GMRES" : "classical", o.maxit);
     if (o.skip) {
         // main.cpp:440-446
@@ -281,14 +283,14 @@ int main(int argc, char **argv) {
     }
     std::printf("\tInitialization = %.4f sec\n", now_s() - t);
 
-    std::vector<int> sched(n / b, fp8 ? 1 : 0);
+    std::vector<int> sched(n / b, fp8 ? 1 : (fp32 ? -1 : 0));
     if (learned && omx_solver_set_policy(s, params.data()) != 0) {
         std::fprintf(stderr, "cannot set the policy\n");
         return 1;
     }
     double tim[5] = {0};
     const double start = now_s();
-    int info = omx_lu_factor(s, A, lda, (fp8 && !learned) ? sched.data() : nullptr, tim);
+    int info = omx_lu_factor(s, A, lda, ((fp8 || fp32) && !learned) ? sched.data() : nullptr, tim);
     const double t_lu = now_s() - start;
     if (learned) {
         std::vector<int> used(n / b, 0);

@@ -607,11 +607,12 @@ void orc_gemm_f8(int64_t m, int64_t n, int64_t k, float alpha,
  * with ld = rows.  For FP8 the per-(tile x tile) exponents live in exps. */
 typedef struct {
     int64_t rows, b;
-    int fp8;
+    int fp8;  /* 1: FP8 e4m3 with tile exponents; -1: FP32 as factored; 0: FP16 */
     int64_t tile;
     uint16_t *h;
     uint8_t *q;
     int32_t *exps;
+    float *f;
 } orc_panel_t;
 
 static void panel_alloc(orc_panel_t *p, int64_t rows_max, int64_t b, int64_t tile) {
@@ -623,12 +624,14 @@ static void panel_alloc(orc_panel_t *p, int64_t rows_max, int64_t b, int64_t til
     p->q = (uint8_t *)malloc((size_t)(rows_max * b + 1));
     const int64_t tmax = (rows_max + tile - 1) / tile * ((b + tile - 1) / tile) + 1;
     p->exps = (int32_t *)malloc(sizeof(int32_t) * (size_t)tmax);
+    p->f = (float *)malloc(sizeof(float) * (size_t)(rows_max * b + 1));
 }
 
 static void panel_free(orc_panel_t *p) {
     free(p->h);
     free(p->q);
     free(p->exps);
+    free(p->f);
 }
 
 /* Fill a panel from FP32 source S(r, l) = src[r*rs + l*ls], r < rows,
@@ -649,6 +652,12 @@ static void panel_fill(orc_panel_t *p, int64_t rows, const float *src, int64_t r
             if (v > mx) mx = v;
         }
     if (maxabs_out) *maxabs_out = mx;
+    if (fp8 < 0) {
+#pragma omp parallel for schedule(static)
+        for (int64_t l = 0; l < b; ++l)
+            for (int64_t r = 0; r < rows; ++r) p->f[r + l * rows] = src[r * rs + l * ls];
+        return;
+    }
     if (!fp8) {
 #pragma omp parallel for schedule(static)
         for (int64_t l = 0; l < b; ++l)
@@ -683,6 +692,22 @@ static void panel_schur(const orc_panel_t *Lp, int64_t lr0, const orc_panel_t *R
     const int64_t b = Lp->b;
     if (!Lp->fp8) {
         orc_gemm_f16(mr, nc, b, -1.0f, Lp->h + lr0, Lp->rows, Rp->h + rc0, Rp->rows, 1.0f, C, ldc);
+        return;
+    }
+    if (Lp->fp8 < 0) {
+        /* FP32 class: the factored panels as they are, the same sequential
+         * FP32 accumulation */
+        float *Xf = (float *)malloc(sizeof(float) * (size_t)(mr * b + 1));
+        float *Yt = (float *)malloc(sizeof(float) * (size_t)(nc * b + 1));
+#pragma omp parallel for schedule(static)
+        for (int64_t l = 0; l < b; ++l)
+            for (int64_t i = 0; i < mr; ++i) Xf[i + l * mr] = Lp->f[lr0 + i + l * Lp->rows];
+#pragma omp parallel for schedule(static)
+        for (int64_t j = 0; j < nc; ++j)
+            for (int64_t l = 0; l < b; ++l) Yt[l + j * b] = Rp->f[rc0 + j + l * Rp->rows];
+        gemm_seq_core(mr, nc, b, -1.0f, Xf, Yt, 1.0f, C, ldc);
+        free(Xf);
+        free(Yt);
         return;
     }
     /* FP8: decode both panels with their tile exponents, then the same
@@ -750,7 +775,8 @@ int orc_lu_factor(int64_t n, int64_t b, int alt, float *A, int64_t lda,
         const int64_t kb = k * b;
         const int64_t rest = n - kb - b; /* rows/cols after block k */
         float *Akk = AT(kb, kb);
-        int fp8 = (sched && sched[k]) ? 1 : 0;
+        /* the step's Schur class: 1 FP8, -1 FP32, 0 FP16 */
+        int fp8 = (sched && sched[k]) ? (sched[k] < 0 ? -1 : 1) : 0;
         if (k > 0) {
             /* diagonal block update with the previous panels (:791-795) */
             panel_schur(&Lprev, 0, &Rprev, 0, b, b, Akk, lda);
@@ -773,8 +799,10 @@ int orc_lu_factor(int64_t n, int64_t b, int alt, float *A, int64_t lda,
             float *Arow = AT(kb, kb + b);
             /* column panel */
             if (k > 0) panel_schur(&Lprev, b, &Rprev, 0, rest, b, Acol, lda);
-            if (k == 0 || alt == 0) {
-                orc_trsm_runn(rest, b, (k == 0) ? Akk : piv, (k == 0) ? lda : b, Acol, lda);
+            if (k == 0 || alt == 0 || fp8 < 0) {
+                /* the factored block itself (piv is its copy for -alt 0, its
+                 * inverses for -alt 2) */
+                orc_trsm_runn(rest, b, Akk, lda, Acol, lda);
             } else {
                 /* W = U^-1, lower part zero */
                 if (alt == 1) {
@@ -801,8 +829,8 @@ int orc_lu_factor(int64_t n, int64_t b, int alt, float *A, int64_t lda,
 
             /* row panel */
             if (k > 0) panel_schur(&Lprev, 0, &Rprev, b, b, rest, Arow, lda);
-            if (k == 0 || alt == 0) {
-                orc_trsm_llnu(b, rest, (k == 0) ? Akk : piv, (k == 0) ? lda : b, Arow, lda);
+            if (k == 0 || alt == 0 || fp8 < 0) {
+                orc_trsm_llnu(b, rest, Akk, lda, Arow, lda);
             } else {
                 if (alt == 1) {
                     for (int64_t j = 0; j < b; ++j)

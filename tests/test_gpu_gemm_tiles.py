@@ -1,6 +1,6 @@
-"""The two trailing-update GEMM kernel families against each other: 256 x 128
-tiles two workgroups per CU (the default) and 256 x 256 tiles one per CU
-(OMX_GEMM_TILE=256).  Both accumulate each C element over k in the same
+"""The two trailing-update GEMM kernel families against each other: 256 x 256
+tiles one workgroup per CU (the default) and 256 x 128 tiles two per CU
+(OMX_GEMM_TILE=half).  Both accumulate each C element over k in the same
 order with the same MFMA instructions, so their results are bit-identical;
 edge tiles (m, n not multiples of the tile), short products (the FP16
 kernel's checked path at k = 64), scaled FP16, beta = 0 and the FP8 scale
@@ -22,7 +22,7 @@ def _L():
 def _run_both(monkeypatch, launch, C0):
     outs = []
     for tile in ("256", "half"):
-        if tile == "half":
+        if tile == "256":
             monkeypatch.delenv("OMX_GEMM_TILE", raising=False)
         else:
             monkeypatch.setenv("OMX_GEMM_TILE", tile)

@@ -81,7 +81,8 @@ def test_lowrank_gemv_matches_dense(orc):
 
 
 @pytest.mark.parametrize("kappa,mode", [(1e2, "fp16"), (1e4, "fp16"), (1e2, "fp8"), (1e4, "fp8"), (1e4, "fp16s"),
-                                        (1e6, "fp16"), (1e6, "fp8"), (1e6, "fp16s")])
+                                        (1e6, "fp16"), (1e6, "fp8"), (1e6, "fp16s"), (1e4, "fp32"),
+                                        (1e6, "fp32")])
 def test_lu_gmres_ir_on_randsvd(orc, kappa, mode):
     """No-pivot LU (FP16, FP16 with per-strip scales, or FP8 Schur) +
     GMRES-IR on a randsvd system.
@@ -92,7 +93,9 @@ def test_lu_gmres_ir_on_randsvd(orc, kappa, mode):
     overflow), and FP8 (3 mantissa bits) is beyond it on these
     non-dominant matrices at any kappa (the oracle's FP8 factors of the
     kappa = 100 case have a backward error ~35 |A|): the solve must then
-    report non-convergence, never a false success."""
+    report non-convergence, never a false success.  The FP32 Schur class
+    (schedule -1 on every step; kappa * u32 <= 0.06) must converge at both
+    kappa = 1e4 and 1e6."""
     torch = torch_cuda()
     L = _L()
     n, b = 2048, 256
@@ -104,7 +107,7 @@ def test_lu_gmres_ir_on_randsvd(orc, kappa, mode):
     h = C.c_void_p()
     flags = 1 if fp8 else (2 if mode == "fp16s" else 0)  # OMX_SOLVER_FP8 / OMX_SOLVER_SCALE16
     assert L.omx_solver_create(n, b, 1, flags, C.byref(h)) == 0
-    sched = np.full(n // b, 1 if fp8 else 0, dtype=np.int32)
+    sched = np.full(n // b, 1 if fp8 else (-1 if mode == "fp32" else 0), dtype=np.int32)
     tim = np.zeros(5)
     info = L.omx_lu_factor(h, ptr(M.A), n, sched.ctypes.data, tim.ctypes.data)
     x = empty(n, torch.float64)
@@ -115,7 +118,7 @@ def test_lu_gmres_ir_on_randsvd(orc, kappa, mode):
                                out.ctypes.data, trace.ctypes.data)
     L.omx_solver_destroy(h)
     print(f"  info {info} rc {rc} steps {int(out[2])} inner {int(out[3])} harness {out[1]:.4g}", flush=True)
-    if kappa <= 1e4 and not fp8:
+    if (kappa <= 1e4 and not fp8) or mode == "fp32":
         assert info == 0 and rc == 0 and out[1] < 16.0
     if rc != 0:
         assert rc == -4 and out[2] == -1  # OMX_ERR_NOCONV, no corrections claimed

@@ -269,6 +269,44 @@ def test_lu_group4_mixed_precision_schedule():
     assert np.max(d2[~low] / (np.abs(facs[0][~low]) + 0.05)) <= 2.0**-4
 
 
+@pytest.mark.parametrize("alt,group4,mixed", [(0, False, False), (1, False, False), (2, False, False),
+                                               (1, True, False), (2, True, True), (1, False, True)])
+def test_lu_fp32_class_matches_oracle(orc, alt, group4, mixed):
+    """Schedule value -1: the step's trailing products run in FP32 on its
+    panels as they stand in A, and its row panel comes from the FP32 TRSM
+    (no FP16 inverse).  All steps in FP32: the factors agree with the
+    oracle's (same schedule, reference step order) to FP32 accumulation
+    differences and their backward error is at FP32 level, far below the
+    FP16 schedule's; a schedule mixing -1 and 0 (runs of one class per
+    group product) agrees to the FP16 tolerance."""
+    torch = torch_cuda()
+    n, b, seed = 2048, 256, 7
+    nb = n // b
+    sched = np.array([-1, 0, 0, -1, -1, 0, -1, 0] if mixed else [-1] * nb, dtype=np.int32)
+    print(f"lu fp32 class n={n} b={b} alt={alt} group4={group4} sched={sched.tolist()}", flush=True)
+    lda = n + 64
+    A, d, rhs = _generate(n, seed, lda)
+    S = Solver(n, b, alt, group4=group4)
+    info, _ = S.lu(A, lda, sched=sched)
+    S.close()
+    assert info == 0
+    got = colmajor_to_numpy(A, n, n, lda).astype(np.float64)
+    Ao, _ = orc.matgen_full(n, seed)
+    want, oinfo = orc.lu_factor(Ao, b, alt=alt, sched=sched)
+    assert oinfo == 0
+    want = want.astype(np.float64)
+    low = np.tril(np.ones((n, n)), -1).astype(bool)
+    dl = np.abs(got - want)[low].max() / np.abs(want[low]).max()
+    du = np.max(np.abs(got - want)[~low] / (np.abs(want[~low]) + 0.05))
+    Lf = np.tril(got, -1) + np.eye(n)
+    Uf = np.triu(got)
+    berr = np.max(np.abs(Lf @ Uf - Ao.astype(np.float64)) / (np.abs(Lf) @ np.abs(Uf)))
+    print(f"  dL {dl:.3e} dU {du:.3e} backward {berr:.3e}", flush=True)
+    tol = 2.0**-8 if mixed else 2.0**-16
+    assert dl <= tol and du <= tol
+    assert berr < (2.0**-9 if mixed else 2.0**-18)
+
+
 @pytest.mark.parametrize("alt,method", [(1, 0), (1, 1), (0, 0)])
 def test_phase_timers(alt, method):
     """With OMX_SOLVER_TIMERS the solver reports GPU time per phase under the

@@ -253,3 +253,53 @@ def test_lu_factor_recovers_known_factors_and_catches_a_dropped_block(orc, alt):
     Ab[512:768, 512:768] += L0[512:768, 0:256] @ U0[0:256, 512:768]
     LUb, _ = orc.lu_factor(Ab.astype(np.float32), b, alt=alt)
     assert np.abs(np.triu(LUb.astype(np.float64)) - U0).max() > 0.05
+
+
+def test_lu_factor_fp32_class_recovers_factors_to_fp32(orc):
+    """Schedule value -1 (the FP32 Schur class): panels by FP32 TRSM, Schur
+    products of the unrounded FP32 panels.  On A = L0 U0 the unit lower
+    factor comes back to FP32 accuracy, more than 50 times closer than with
+    the FP16 class (which rounds every Schur operand to 11 bits), and a mixed
+    schedule sits in between; U's error is the rounding of A to FP32 either
+    way."""
+    n, b = 1280, 256
+    rng = np.random.default_rng(77)
+    L0 = np.tril(rng.uniform(0, 1, (n, n)) / n, -1) + np.eye(n)
+    U0 = np.triu(rng.uniform(0, 1, (n, n)), 1)
+    U0 += np.diag(1.0 + U0.sum(axis=1))
+    A = (L0 @ U0).astype(np.float32)
+    errs = {}
+    for name, sched in (("fp32", [-1] * 5), ("fp16", None), ("mixed", [-1, 0, -1, 0, 0])):
+        for alt in (0, 2):
+            LU, info = orc.lu_factor(A, b, alt=alt, sched=sched)
+            assert info == 0
+            Ug = np.triu(LU.astype(np.float64))
+            Lg = np.tril(LU.astype(np.float64), -1) + np.eye(n)
+            errs[name, alt] = (np.abs(Ug - U0).max() / np.abs(U0).max(), np.abs(Lg - L0).max())
+    for alt in (0, 2):
+        eu32, el32 = errs["fp32", alt]
+        eu16, el16 = errs["fp16", alt]
+        _, elm = errs["mixed", alt]
+        assert eu32 < 4e-6 and eu16 < 4e-6 and el32 < 1e-7, errs
+        assert el16 > 50 * el32, errs
+        assert el32 < elm < el16 * 1.01, errs
+
+
+def test_lu_factor_fp32_class_resolves_what_fp16_cannot(orc):
+    """randsvd with kappa_2 = 1e4 (8 reflectors per factor): the FP16 Schur
+    class perturbs the LU by ~u16 |L||U| ~ 5e-4, more than the smallest
+    singular value 1e-4, so its factors are no preconditioner; the FP32 class
+    keeps the backward error two orders below sigma_min."""
+    n, b = 1024, 256
+    A64 = orc.randsvd_dense(n, 42, 1e4, 8)
+    A = A64.astype(np.float32)
+    res = {}
+    for name, sched in (("fp32", [-1] * 4), ("fp16", [0] * 4)):
+        LU, info = orc.lu_factor(A, b, alt=1, sched=sched)
+        F = LU.astype(np.float64)
+        L = np.tril(F, -1) + np.eye(n)
+        U = np.triu(F)
+        res[name] = np.abs(L @ U - A64).max() if info == 0 else np.inf
+    smin = np.linalg.svd(A64, compute_uv=False)[-1]
+    assert res["fp32"] < 1e-2 * smin, (res, smin)
+    assert res["fp16"] > smin, (res, smin)
