@@ -301,10 +301,18 @@ def test_lu_fp32_class_matches_oracle(orc, alt, group4, mixed):
     Lf = np.tril(got, -1) + np.eye(n)
     Uf = np.triu(got)
     berr = np.max(np.abs(Lf @ Uf - Ao.astype(np.float64)) / (np.abs(Lf) @ np.abs(Uf)))
-    print(f"  dL {dl:.3e} dU {du:.3e} backward {berr:.3e}", flush=True)
+    Lo = np.tril(want, -1) + np.eye(n)
+    Uo = np.triu(want)
+    bo = np.max(np.abs(Lo @ Uo - Ao.astype(np.float64)) / (np.abs(Lo) @ np.abs(Uo)))
+    print(f"  dL {dl:.3e} dU {du:.3e} backward {berr:.3e} (oracle {bo:.3e})", flush=True)
     tol = 2.0**-8 if mixed else 2.0**-16
     assert dl <= tol and du <= tol
-    assert berr < (2.0**-9 if mixed else 2.0**-18)
+    # componentwise backward error: the oracle's under the same schedule
+    # (FP16 steps: ~2.5e-3 at this seed, also with every step in FP16), and
+    # FP32 level when every step is FP32
+    assert berr <= 1.5 * bo + 2.0**-20
+    if not mixed:
+        assert berr < 2.0**-18
 
 
 @pytest.mark.parametrize("alt,method", [(1, 0), (1, 1), (0, 0)])
