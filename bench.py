@@ -218,8 +218,9 @@ def randsvd_table(n, b, alt, kappas, policy, seed, group=4, nrefl=8, fp32=True):
     """The conditioned systems: randsvd A = U diag(sigma) V^T (sigma
     geometric from 1 to 1/kappa, U and V products of nrefl seeded Householder
     reflectors) for each kappa, solved with the Schur update in FP16 on every
-    step, FP8 on every step, and the learned schedule (FP16 fallback when its
-    refinement fails), all with groups of `group` panels and GMRES-IR; where
+    step, FP8 on every step, and the learned schedule (when its refinement
+    fails: FP16, unless the learned schedule was all FP16, then the FP32
+    class), all with groups of `group` panels and GMRES-IR; where
     kappa * 2^-11 >= 1 (beyond FP16) also with every step in the FP32 class
     (schedule -1: FP32 trailing products on the panels as they stand).  The
     forward error is against the same solution refined on with three more
@@ -269,13 +270,19 @@ def randsvd_table(n, b, alt, kappas, policy, seed, group=4, nrefl=8, fp32=True):
             L.omx_solver_last_schedule(h, used.ctypes.data, None)
             fp8_steps = int((used[:-1] > 0).sum())
             ok = info == 0 and rc == 0 and out[1] < 16.0
-            fell_back = False
+            fell_back = None
             if meth == "learned" and not ok:
-                fell_back = True
-                sysm.generate()
-                info, rc, out, _ = lu_ir([0] * nb)
-                torch.cuda.synchronize()
-                ok = info == 0 and rc == 0 and out[1] < 16.0
+                # the fallback ladder: FP16 (skipped when the learned
+                # schedule had no FP8 step, i.e. was the FP16 schedule), then
+                # the FP32 class
+                for rung, sv in (("fp16", 0), ("fp32", -1)):
+                    if ok or (rung == "fp16" and fp8_steps == 0):
+                        continue
+                    fell_back = rung
+                    sysm.generate()
+                    info, rc, out, _ = lu_ir([sv] * nb)
+                    torch.cuda.synchronize()
+                    ok = info == 0 and rc == 0 and out[1] < 16.0
             t = time.perf_counter() - t0
             fwd = None
             if info == 0:
@@ -291,10 +298,10 @@ def randsvd_table(n, b, alt, kappas, policy, seed, group=4, nrefl=8, fp32=True):
                    "tflops": round(flops / t * 1e-12, 2), "harness": float(out[1]), "corrections": int(out[2]),
                    "inner_iters": int(out[3]), "forward_error": fwd, "fp8_steps": fp8_steps, "steps": nb - 1}
             if meth == "learned":
-                row["fell_back_to_fp16"] = fell_back
+                row["fell_back_to"] = fell_back
             log(f"[randsvd kappa={kappa:g} {meth}] {t:.3f} s converged {ok} harness {out[1]:.4g} corrections "
                 f"{int(out[2])} inner {int(out[3])} forward error {fwd} FP8 steps {fp8_steps}/{nb - 1}"
-                + (" [fell back to FP16]" if fell_back else ""))
+                + (f" [fell back to {fell_back}]" if fell_back else ""))
             rows.append(row)
         del sysm, x
         torch.cuda.empty_cache()

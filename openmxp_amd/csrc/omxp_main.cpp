@@ -6,9 +6,10 @@
 // (a conditioned matrix of 2-norm condition number kappa).  One GPU, 1x1
 // grid.  With -prec learned the MLP policy picks each step's Schur
 // precision; -prec fp32 runs every trailing update in FP32 (panels read
-// straight from A, for systems whose kappa * 2^-11 exceeds 1); whenever FP8 steps were used (-prec fp8 or learned) and the
-// refinement then fails, the system is generated again and factored in FP16
-// (inside the timed region).  -skip is the reference's bit-mask (IR, GETRF,
+// straight from A, for systems whose kappa * 2^-11 exceeds 1).  With -prec
+// fp8 or learned, when the refinement fails the system is generated again
+// and factored in FP16 (skipped when the learned schedule was all FP16), and
+// if that fails too in FP32, inside the timed region.  -skip is the reference's bit-mask (IR, GETRF,
 // TRSM, GEMM; main.cpp:68-70); -progress k prints the reference's
 // projected-rate line for every k-th step (panel_trf_gdirect.hpp:1146-1161)
 // from per-step completion events, after the factorisation.  The timed region is LU + IR, generation
@@ -331,16 +332,27 @@ int main(int argc, char **argv) {
                                  trace.data());
     }
     if (do_ir && fp8 && rc != 0) {
-        // fallback: the factors with FP8 Schur steps did not refine; the
-        // system again, all steps in FP16
-        std::printf("FP8 schedule did not converge: regenerating and factoring in FP16\n");
-        if (generate()) return 1;
-        CK(hipDeviceSynchronize());
-        std::fill(sched.begin(), sched.end(), 0);
-        info = omx_lu_factor(s, A, lda, sched.data(), tim);
-        rc = omx_ir_solve_system(s, A, lda, &sys, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, 16.0, 0, out,
-                                 trace.data());
-        std::printf("fallback: FP16 factorisation, refinement %s\n", rc == 0 ? "converged" : "did not converge");
+        // the fallback ladder: the factors with FP8 Schur steps did not
+        // refine; the system again with every step in FP16 (skipped when the
+        // learned schedule chose FP16 for every step), then in the FP32 class
+        int n8 = 0;
+        if (learned) {
+            std::vector<int> used(n / b, 0);
+            omx_solver_last_schedule(s, used.data(), nullptr);
+            for (int v : used) n8 += v > 0;
+        }
+        const bool try16 = !learned || n8 > 0;
+        for (int rung = try16 ? 0 : -1; rung >= -1 && rc != 0; --rung) {
+            const char *name = rung == 0 ? "FP16" : "FP32";
+            std::printf("schedule did not converge: regenerating and factoring in %s\n", name);
+            if (generate()) return 1;
+            CK(hipDeviceSynchronize());
+            std::fill(sched.begin(), sched.end(), rung);
+            info = omx_lu_factor(s, A, lda, sched.data(), tim);
+            rc = omx_ir_solve_system(s, A, lda, &sys, rhs, x, o.ir, o.maxit, o.restart, o.inner_tol, 16.0, 0, out,
+                                     trace.data());
+            std::printf("fallback: %s factorisation, refinement %s\n", name, rc == 0 ? "converged" : "did not converge");
+        }
     }
     const double stop = now_s();
     const int steps = !do_ir ? 0 : (rc == 0) ? (int)out[2] + 1 : o.maxit;

@@ -179,7 +179,28 @@ def test_fp8_failure_falls_back_to_fp16():
     torch_cuda()
     r = run(2048, 256, "-randsvd", "1e2", "-prec", "fp8", "-ir", "gmres", "-maxit", "10")
     print(r.stdout[-900:], flush=True)
-    assert "FP8 schedule did not converge" in r.stdout
+    assert "schedule did not converge: regenerating and factoring in FP16" in r.stdout
     assert "fallback: FP16 factorisation, refinement converged" in r.stdout
+    assert "FP32" not in r.stdout.split("fallback: FP16")[1]
+    assert r.returncode == 0, r.stderr
+    assert _harness(r.stdout) < 16.0
+
+
+@pytest.mark.gpu
+def test_fallback_ladder_reaches_fp32():
+    """randsvd kappa 1e6 (kappa * 2^-11 ~ 500: beyond FP16): the FP8 factors
+    and then the FP16 factors do not refine; the third rung, every step in
+    the FP32 class, does (harness < 16).  -prec fp32 alone refines too."""
+    from tests.gpu_util import torch_cuda
+    torch_cuda()
+    r = run(2048, 256, "-randsvd", "1e6", "-prec", "fp8", "-ir", "gmres", "-maxit", "10")
+    print(r.stdout[-1200:], flush=True)
+    assert "fallback: FP16 factorisation, refinement did not converge" in r.stdout
+    assert "regenerating and factoring in FP32" in r.stdout
+    assert "fallback: FP32 factorisation, refinement converged" in r.stdout
+    assert r.returncode == 0, r.stderr
+    assert _harness(r.stdout) < 16.0
+    r = run(2048, 256, "-randsvd", "1e6", "-prec", "fp32", "-ir", "gmres")
+    assert "Schur precision FP32" in r.stdout
     assert r.returncode == 0, r.stderr
     assert _harness(r.stdout) < 16.0
