@@ -552,7 +552,7 @@ int omx_solver_step_times(omx_solver *s, double *ms);
 int omx_solver_set_policy(omx_solver *s, const float *params);
 
 /* The last factorisation's per-step choices (sched, n/b ints: 0 FP16,
- * 1 + h FP8) and, for FP8 solvers, the 16 scheduler features of every step
+ * 1 + h FP8, -1 FP32) and, for FP8 solvers, the 16 scheduler features of every step
  * (features, n/b x 16 floats, row-major; zero for the last step, which has
  * no panels).  Either pointer may be NULL. */
 int omx_solver_last_schedule(omx_solver *s, int *sched, float *features);

@@ -309,7 +309,9 @@ int main(int argc, char **argv) {
                 const double fact = 1.0 - (double)(k * b) / dn;
                 const double left = 100.0 * fact * fact * fact;
                 const double used_s = ms[k] * 1e-3;
-                const double proj_l = left < 100.0 ? used_s / (100.0 - left) * left : 0.0;
+                // as the reference: step 0 leaves 100 % of the work, so its
+                // projection is infinite (and its rate 0)
+                const double proj_l = used_s / (100.0 - left) * left;
                 const double proj_t = used_s + proj_l;
                 const double gf = proj_t > 0 ? 2.0 / 3.0 * dn * dn * dn / proj_t * 1.0e-09 : 0.0;
                 std::printf("Step %ld, Rank 0, Elapsed %.2lf, Work left %.2lf%%, Time left: %.2lf, Est Total: "
@@ -383,6 +385,10 @@ int main(int argc, char **argv) {
             const int gu = 10, gp = 11;  // GEMM_UPDATE, GEMM_PROGRESS
             const double gt = sec[gu] + sec[gp];
             std::printf("GEMM_TOTAL: %f Tflops\n", gt > 0 ? 1e-12 * (ops[gu] + ops[gp]) / gt : 0.0);
+            if (sec[OMX_TIMER_ITEMS - 1] < 0)
+                std::printf("(MISC = TOTAL - the items is negative: the trailing updates on the bulk stream run "
+                            "beside the factorisation's items, %.3f s of overlap)\n",
+                            -sec[OMX_TIMER_ITEMS - 1]);
         }
     }
     omx_solver_destroy(s);

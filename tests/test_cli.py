@@ -120,6 +120,8 @@ def test_skip_ir_and_progress():
     el = [float(l.split("Elapsed ")[1].split(",")[0]) for l in steps]
     left = [float(l.split("Work left ")[1].split("%")[0]) for l in steps]
     assert el == sorted(el) and left[0] == 100.0 and left == sorted(left, reverse=True)
+    # step 0 leaves all the work: the reference's projection is infinite there
+    assert "Time left: inf" in steps[0] and "GFlops: 0.00" in steps[0]
 
 
 def _ir_steps(out):
