@@ -85,8 +85,11 @@ __device__ __forceinline__ void wait_stages(int stages) {
 // wait the compiler puts before each MFMA completes the reads of a ring slot
 // before any wave passes the barrier after which that slot is refilled.  The
 // memory clobber keeps the compiler from moving LDS accesses across it.
-// Used by the FP16 kernel (+3 % on the trailing update); the FP8 kernel
-// measured the same either way and keeps __syncthreads().
+// Used by both 256 x 256 kernels: +3 % on the FP16 trailing update; in the
+// FP8 kernel, whose stages come in pairs, __syncthreads() made the second
+// stage of each pair wait for every copy in flight (the refill issued eight
+// MFMAs earlier included), a prefetch distance of one stage: +6 % (k = 10240)
+// to +10 % (k = 5120) without it.
 __device__ __forceinline__ void stage_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
 __device__ __forceinline__ void gl_lds16(const char *g, char *l) {
@@ -428,18 +431,16 @@ k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *
 
     for (int st = 0; st < kNbuf - 1 && st < nks; ++st) issue(st);
     wait_stages(min(kNbuf - 2, nks - 1));
-    __syncthreads();
+    stage_barrier();
     intx8 xa[4], ya[4], xb[4], yb[4];
     frags(xa, ya, 0);
     // E8M0 scales: value = fp8 * 2^(s - 127) with s = 127 - e.  One stage:
     // 8 MFMAs; the wait for the next stage and the stage barrier; 8 MFMAs
     // with the refill of the freed ring slot and the next stage's fragment
     // reads between them (kSteady: as for FP16).
-    auto body = [&](intx8(&xc)[4], intx8(&yc)[4], intx8(&xn)[4], intx8(&yn)[4], int cs, auto steady_tag) {
+    auto body = [&](intx8(&xc)[4], intx8(&yc)[4], intx8(&xn)[4], intx8(&yn)[4], int cs, int sx, int sy,
+                    auto steady_tag) {
         constexpr bool kSteady = decltype(steady_tag)::value;
-        const int kt = (cs * 64) / OMX_SCALE_TILE;
-        const int sx = 127 - xe[kt];
-        const int sy = 127 - ye[kt];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi)
@@ -451,10 +452,10 @@ k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *
         const bool more = kSteady || cs + 1 < nks;
         if (kSteady) {
             wait_stages(kNbuf - 3);
-            __syncthreads();
+            stage_barrier();
         } else if (more) {
             wait_stages(min(kNbuf - 3, nks - cs - 2));
-            __syncthreads();
+            stage_barrier();
         }
         __builtin_amdgcn_sched_barrier(0);
         if (kSteady) {
@@ -484,15 +485,25 @@ k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *
                                                                               sy, 0, sx);
         __builtin_amdgcn_sched_barrier(0);
     };
-    // nks is a multiple of 4 (k % 256 == 0): stages come in pairs
+    // nks is a multiple of 4 (k % 256 == 0): stages come in pairs, both in
+    // one 256-column scale tile; the next pair's scale exponents are loaded
+    // at the top of a pair, so no stage waits on a scalar load
     int st = 0;
+    int sx = 127 - xe[0], sy = 127 - ye[0];
     for (; st + kNbuf < nks; st += 2) {
-        body(xa, ya, xb, yb, st, std::true_type{});
-        body(xb, yb, xa, ya, st + 1, std::true_type{});
+        const int nt = ((st + 2) * 64) / OMX_SCALE_TILE;
+        const int sxn = 127 - xe[nt], syn = 127 - ye[nt];
+        body(xa, ya, xb, yb, st, sx, sy, std::true_type{});
+        body(xb, yb, xa, ya, st + 1, sx, sy, std::true_type{});
+        sx = sxn;
+        sy = syn;
     }
     for (; st < nks; st += 2) {
-        body(xa, ya, xb, yb, st, std::false_type{});
-        body(xb, yb, xa, ya, st + 1, std::false_type{});
+        const int kt = (st * 64) / OMX_SCALE_TILE;
+        sx = 127 - xe[kt];
+        sy = 127 - ye[kt];
+        body(xa, ya, xb, yb, st, sx, sy, std::false_type{});
+        body(xb, yb, xa, ya, st + 1, sx, sy, std::false_type{});
     }
     epilogue<kBetaZero>(acc, M, N, alpha, beta, C, ldc, tm, tn, wm, wn, lane);
 }
