@@ -88,6 +88,9 @@ def parse():
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the Schur GEMM traffic from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
                          "counter_collection files of a --copy-calibration run, and exit")
+    ap.add_argument("--summarize-sq", metavar="SQ_CSV",
+                    help="print clock, MFMA busy, waves per SIMD and wait fractions of the trailing-update GEMMs "
+                         "from a rocprofv3 --pmc counter_collection file of a --gemm-sweep run, and exit")
     return ap.parse_args()
 
 
@@ -414,6 +417,45 @@ def pmc_traffic(tag, n, b, group):
     return float(m.group(1)) if m else None
 
 
+SQ_COUNTERS = ("GRBM_GUI_ACTIVE", "SQ_BUSY_CYCLES", "SQ_WAVE_CYCLES", "SQ_VALU_MFMA_BUSY_CYCLES", "SQ_WAIT_ANY",
+               "SQ_WAIT_INST_LDS", "SQ_ACTIVE_INST_LDS", "SQ_LDS_BANK_CONFLICT")
+
+
+def summarize_sq(sq_csv):
+    """Shader-sequencer counters of the sweep's beta = 1 trailing-update GEMMs
+    (one rocprofv3 --pmc pass with SQ_COUNTERS): per kernel family the sums
+    and clock = GRBM_GUI_ACTIVE / 8 XCDs / kernel time, MFMA busy =
+    SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / 8 x 1024 SIMDs), waves per
+    SIMD = 4 SQ_WAVE_CYCLES / (GRBM_GUI_ACTIVE / 8 x 1024), waiting =
+    SQ_WAIT_ANY / SQ_WAVE_CYCLES (LDS: SQ_WAIT_INST_LDS / SQ_WAVE_CYCLES)."""
+    import collections
+    import csv
+    rows = list(csv.DictReader(open(sq_csv)))
+    for pre in ("k_gemm_f16<false", "k_gemm_f8<false"):
+        disp = {}
+        for r in rows:
+            if pre not in r["Kernel_Name"]:
+                continue
+            d = disp.setdefault(r["Dispatch_Id"], {"dur": int(r["End_Timestamp"]) - int(r["Start_Timestamp"])})
+            d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        if not disp:
+            continue
+        tot = collections.Counter()
+        dur = 0
+        for d in disp.values():
+            dur += d.pop("dur")
+            tot.update(d)
+        g = tot["GRBM_GUI_ACTIVE"]
+        print(f"{pre}: {len(disp)} launches, {dur / 1e6:.1f} ms")
+        for k in SQ_COUNTERS:
+            print(f"  {k:26s} {tot[k]:.4e}")
+        print(f"  clock {g / 8 / (dur * 1e-9) / 1e9:.3f} GHz; MFMA busy {tot['SQ_VALU_MFMA_BUSY_CYCLES'] / (g / 8 * 1024):.3f}; "
+              f"waves/SIMD {4 * tot['SQ_WAVE_CYCLES'] / (g / 8 * 1024):.2f}; "
+              f"waiting {tot['SQ_WAIT_ANY'] / tot['SQ_WAVE_CYCLES']:.3f} "
+              f"(LDS {tot['SQ_WAIT_INST_LDS'] / tot['SQ_WAVE_CYCLES']:.3f}); "
+              f"LDS issuing {tot['SQ_ACTIVE_INST_LDS'] / tot['SQ_WAVE_CYCLES']:.3f}")
+
+
 def summarize_pmc(fetch_csv, write_csv, n, b):
     """Schur GEMM HBM traffic per launch (FP16 and FP8 trailing updates) from
     two rocprofv3 PMC runs of the GEMM sweep, each counter scaled by what it
@@ -612,6 +654,9 @@ def main():
         return
     if args.summarize_pmc:
         summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.n, args.b)
+        return
+    if args.summarize_sq:
+        summarize_sq(args.summarize_sq)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
