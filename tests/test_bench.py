@@ -50,3 +50,27 @@ def test_small_run_prints_the_contract_line():
     assert roof["traffic"] is None  # measured at n = 204800 only
     cpu = line["cpu_baseline"]
     assert cpu["value"] > 0 and cpu["kind"] in ("port", "reference") and cpu["cores"] >= 1
+
+
+def test_sq_summary_derives_clock_busy_and_waits(tmp_path, capsys):
+    """--summarize-sq on a synthetic counter file: one FP8 trailing-update
+    dispatch of 1 ms at 2 GHz on 8 XCDs with the matrix pipes of its 1024
+    SIMDs half busy, one wave per SIMD, a quarter of the wave cycles waiting
+    (an eighth on LDS)."""
+    sys.path.insert(0, ROOT)
+    import bench
+    cyc = 2.0e6  # 1 ms at 2 GHz
+    vals = {"GRBM_GUI_ACTIVE": 8 * cyc, "SQ_BUSY_CYCLES": 8 * cyc, "SQ_WAVE_CYCLES": 1024 * cyc / 4,
+            "SQ_VALU_MFMA_BUSY_CYCLES": 0.5 * 1024 * cyc, "SQ_WAIT_ANY": 0.25 * 1024 * cyc / 4,
+            "SQ_WAIT_INST_LDS": 0.125 * 1024 * cyc / 4, "SQ_ACTIVE_INST_LDS": 0.0, "SQ_LDS_BANK_CONFLICT": 0.0}
+    path = tmp_path / "sq.csv"
+    with open(path, "w") as f:
+        f.write("Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value,Start_Timestamp,End_Timestamp\n")
+        for k, v in vals.items():
+            f.write(f'7,"void k_gemm_f8<false, 4>(long)",{k},{v},1000000,2000000\n')
+        f.write('8,"k_other(long)",GRBM_GUI_ACTIVE,5,0,10\n')
+    bench.summarize_sq(str(path))
+    out = capsys.readouterr().out
+    assert "k_gemm_f8<false: 1 launches, 1.0 ms" in out
+    assert "clock 2.000 GHz; MFMA busy 0.500; waves/SIMD 1.00; waiting 0.250 (LDS 0.125)" in out
+    assert "k_gemm_f16<false" not in out
