@@ -88,6 +88,13 @@ def parse():
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the Schur GEMM traffic from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
                          "counter_collection files of a --copy-calibration run, and exit")
+    ap.add_argument("--ir-sweep", action="store_true",
+                    help="run only the refinement's kernels (lower and upper triangular solves, residual GEMV) on a "
+                         "generated n x n matrix, three times each, print their timings, and exit (for counter "
+                         "collection; the matrix stands in for the factors, so the values are not a solve)")
+    ap.add_argument("--summarize-ir", nargs=2, metavar=("FETCH_CSV", "SQ_CSV"),
+                    help="print the triangular solves' traffic and the residual GEMV's VALU figures from rocprofv3 "
+                         "--pmc counter_collection files of an --ir-sweep run (FETCH_SIZE; IR_SQ_COUNTERS), and exit")
     ap.add_argument("--summarize-sq", metavar="SQ_CSV",
                     help="print clock, MFMA busy, waves per SIMD and wait fractions of the trailing-update GEMMs "
                          "from a rocprofv3 --pmc counter_collection file of a --gemm-sweep run, and exit")
@@ -357,7 +364,8 @@ def ir_kernels(gs, seed, reps=3):
                        "fp64_frac": round(tf / PEAK_FP64_VALU_TFLOPS, 4),
                        "note": "each element is one LCG step (64-bit integer multiply-add), an int64->FP64 "
                                "conversion and scaling, and one FP64 FMA: the FP64 fraction understates the "
-                               "VALU issue rate (see the SQ counter summary under profiles/)"}
+                               "VALU issue rate (11 VALU instructions per element, SIMDs issuing VALU ~96 % of "
+                               "their cycles: profiles/ir_kernels_n204800.txt)"}
     return out
 
 
@@ -419,6 +427,63 @@ def pmc_traffic(tag, n, b, group):
 
 SQ_COUNTERS = ("GRBM_GUI_ACTIVE", "SQ_BUSY_CYCLES", "SQ_WAVE_CYCLES", "SQ_VALU_MFMA_BUSY_CYCLES", "SQ_WAIT_ANY",
                "SQ_WAIT_INST_LDS", "SQ_ACTIVE_INST_LDS", "SQ_LDS_BANK_CONFLICT")
+
+
+IR_SQ_COUNTERS = ("GRBM_GUI_ACTIVE", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_ACTIVE_INST_VALU", "SQ_INSTS_VALU",
+                  "SQ_INSTS_SALU", "SQ_WAIT_ANY", "SQ_ACTIVE_INST_ANY")
+
+
+def summarize_ir(fetch_csv, sq_csv, n):
+    """The refinement's kernels from the counter passes of an --ir-sweep run:
+    the triangular solves' bytes fetched past L2 (FETCH_SIZE, times the copy
+    calibration factor 2 of the trailing-update summary) against their 2 n^2
+    bytes of factor reads per triangle; the residual GEMV's VALU figures:
+    VALU instructions per matrix element, and the fraction of the SIMDs'
+    cycles with a VALU instruction in flight (SQ_ACTIVE_INST_VALU counts
+    per-wave cycles in quads: 4 x it / (GRBM_GUI_ACTIVE / 8 XCDs x 1024
+    SIMDs))."""
+    import collections
+    import csv
+
+    def per_kernel(path):
+        out = collections.defaultdict(lambda: collections.defaultdict(float))
+        durs = collections.defaultdict(dict)
+        for r in csv.DictReader(open(path)):
+            name = r["Kernel_Name"]
+            key = ("trsv_lower" if "k_trsv_lower" in name else "trsv_upper" if "k_trsv_upper" in name
+                   else "otf_gemv" if "k_otf_gemv" in name else None)
+            if key is None:
+                continue
+            out[key][r["Counter_Name"]] += float(r["Counter_Value"])
+            durs[key][r["Dispatch_Id"]] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        return out, durs
+    fc, fd = per_kernel(fetch_csv)
+    tri = 2.0 * n * n
+    for key in ("trsv_lower", "trsv_upper"):
+        if key not in fc:
+            continue
+        nd = len(fd[key])
+        solves = nd / (n // 256) if n >= 256 else nd
+        fetched = fc[key]["FETCH_SIZE"] * 1024 * 2.0
+        sec = sum(fd[key].values()) * 1e-9
+        print(f"{key}: {nd} launches ({solves:.0f} solves), {sec * 1e3:.2f} ms in all, "
+              f"{tri * solves / sec * 1e-9:.1f} GB/s of factor reads (8000 peak); "
+              f"fetched past L2 {fetched / solves:.4e} bytes per solve = {fetched / (tri * solves):.3f} x the 2 n^2 "
+              "factor bytes")
+    sc, sd = per_kernel(sq_csv)
+    if "otf_gemv" in sc:
+        t = sc["otf_gemv"]
+        nd = len(sd["otf_gemv"])
+        sec = sum(sd["otf_gemv"].values()) * 1e-9
+        g = t["GRBM_GUI_ACTIVE"]
+        elems = float(n) * n * nd
+        print(f"otf_gemv: {nd} launches, {sec * 1e3:.2f} ms in all, {elems / sec:.4e} elements/s; "
+              f"clock {g / 8 / sec * 1e-9:.3f} GHz")
+        for k in IR_SQ_COUNTERS:
+            print(f"  {k:22s} {t[k]:.4e}")
+        print(f"  VALU instructions per element {t['SQ_INSTS_VALU'] * 64 / elems:.2f} (per lane-element: "
+              f"wave instructions x 64 lanes / elements); VALU busy {4 * t['SQ_ACTIVE_INST_VALU'] / (g / 8 * 1024):.3f} "
+              f"of SIMD cycles; waves waiting {t['SQ_WAIT_ANY'] / t['SQ_WAVE_CYCLES']:.3f} of wave cycles")
 
 
 def summarize_sq(sq_csv):
@@ -657,6 +722,14 @@ def main():
         return
     if args.summarize_sq:
         summarize_sq(args.summarize_sq)
+        return
+    if args.summarize_ir:
+        summarize_ir(args.summarize_ir[0], args.summarize_ir[1], args.n)
+        return
+    if args.ir_sweep:
+        gs = GpuSolve(args.n, args.b, args.alt, fp8=False, group=2)
+        gs.generate(args.seed)
+        print(json.dumps({"ir_sweep": ir_kernels(gs, args.seed), "n": args.n, "b": args.b}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
