@@ -4,7 +4,8 @@
   593-1192) at b = 2560: n = 12800 (5 panels, pairs: the trailing update is a
   product of depth 2b = 5120 over the pair buffer's two K windows) and
   n = 23040 (9 panels, groups of four: depth 4b = 10240, 40 FP8 scale tiles
-  per 256-row strip), FP16 and FP8 schedules, -alt 0/1/2.
+  per 256-row strip), FP16 and FP8 schedules, -alt 0/1/2, and the FP32
+  Schur class (all steps, and mixed with FP8 and FP16 steps).
 - The windowed Schur GEMMs at the bench's depths (FP16 K = 5120, FP8
   K = 10240) against the exact product of their rounded operands.
 - Whole solves at N = 61440 and N = 204800 (b = 2560): harness < 16 and the
@@ -121,6 +122,20 @@ def test_lu_b2560_groups4_fp8(orc):
     """The learned solver's configuration (FP8 in groups of four, -alt 2
     masks) with one FP16 panel inside group 1."""
     _factor_vs_oracle(orc, 23040, 2, True, [1, 1, 1, 1, 1, 0, 1, 1, 0])
+
+
+def test_lu_b2560_groups4_fp32_class(orc):
+    """Every step in the FP32 class (schedule -1) at b = 2560 in groups of
+    four: each group's trailing update is one FP32 product of depth 4b =
+    10240 straight from the FP32 panels in A; the factors match the
+    oracle's FP32-class LU."""
+    _factor_vs_oracle(orc, 23040, 1, True, [-1] * 9)
+
+
+def test_lu_b2560_pairs_three_classes(orc):
+    """FP8, FP32 and FP16 steps in one factorisation (pairs at b = 2560):
+    a pair of one FP32 and one FP16 panel is two products, one per class."""
+    _factor_vs_oracle(orc, 12800, 1, False, [1, -1, 0, -1, 0])
 
 
 def _pack_window(L, S, fp8, P, E, ldk, k0, tmax):
