@@ -211,12 +211,17 @@ __device__ __forceinline__ void epilogue(floatx16 (&acc)[4][4], int64_t M, int64
 // vmcnt wait: 16 after D0, then at most 63 outstanding (the copies of the
 // batch after next and part of the stores) for the later batches.  The
 // caller has passed a barrier after the last stage (no wave still reads
-// the ring); C 16-byte aligned, ldc % 4 == 0.
+// the ring); C 16-byte aligned, ldc % 4 == 0.  kRows row blocks of a batch
+// are read back and stored at a time: 4 in the FP8 kernel; 2 in the FP16
+// one, whose main loop leaves fewer registers free (there the refill comes
+// after the first 32 stores of its batch, so the vmcnt(63) waits are
+// stricter than needed, never weaker).
 __device__ __forceinline__ bool c_lds_ok(int64_t M, int64_t N, const float *C, int64_t ldc, int tm, int tn) {
     return (int64_t)(tm + 1) * OMX_STRIP <= M && (int64_t)(tn + 1) * OMX_STRIP <= N &&
            ldc * 32 * 4 < ((int64_t)1 << 31) && ldc % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0;
 }
 
+template <int kRows>
 __device__ __forceinline__ void epilogue_lds(floatx16 (&acc)[4][4], float alpha, float beta,
                                              float *__restrict__ C, int64_t ldc, int64_t i0, int64_t j0,
                                              int lane, int wave, char *ring) {
@@ -233,6 +238,14 @@ __device__ __forceinline__ void epilogue_lds(floatx16 (&acc)[4][4], float alpha,
 #pragma unroll
         for (int t = 0; t < 16; ++t) gl_lds16(g + (int64_t)2 * t * ldc * 4, l + t * 1024);
     };
+    // the half of batch ni is refilled with batch ni + 2 once its reads have
+    // completed
+    auto refill = [&](int ni, int h) {
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+        __builtin_amdgcn_sched_barrier(0);
+        dma(ni + 2, h);
+    };
     dma(0, 0);
     dma(1, 1);
 #pragma unroll
@@ -245,28 +258,44 @@ __device__ __forceinline__ void epilogue_lds(floatx16 (&acc)[4][4], float alpha,
             __builtin_amdgcn_s_waitcnt(0xCF7F);  // vmcnt(63)
         __builtin_amdgcn_sched_barrier(0);
         const char *l = reg + h * 16384;
-        float old[4][16];
+        if constexpr (kRows == 4) {
+            float old[4][16];
 #pragma unroll
-        for (int mi = 0; mi < 4; ++mi)
+            for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
-                old[mi][r] = *reinterpret_cast<const float *>(
-                    l + (8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * 512 + (32 * mi + (lane & 31)) * 4);
-        if (ni + 2 < 4) {
-            // the half is refilled once its reads have completed
-            __builtin_amdgcn_sched_barrier(0);
-            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-            __builtin_amdgcn_sched_barrier(0);
-            dma(ni + 2, h);
+                for (int r = 0; r < 16; ++r)
+                    old[mi][r] = *reinterpret_cast<const float *>(
+                        l + (8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * 512 + (32 * mi + (lane & 31)) * 4);
+            if (ni + 2 < 4) refill(ni, h);
+            const __amdgpu_buffer_rsrc_t rs = colblock_rsrc(C + (j0 + 32 * ni) * ldc + i0, ldc);
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    __builtin_amdgcn_raw_buffer_store_b32(
+                        __builtin_bit_cast(uint32_t, beta * old[mi][r] + alpha * acc[mi][ni][r]), rs,
+                        voff[r] + mi * 128, 0, kCPolicy);
+        } else {
+            const __amdgpu_buffer_rsrc_t rs = colblock_rsrc(C + (j0 + 32 * ni) * ldc + i0, ldc);
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                float old[2][16];
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        old[q][r] = *reinterpret_cast<const float *>(
+                            l + (8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * 512 + (32 * (2 * half + q) + (lane & 31)) * 4);
+                if (half == 1 && ni + 2 < 4) refill(ni, h);
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        __builtin_amdgcn_raw_buffer_store_b32(
+                            __builtin_bit_cast(uint32_t, beta * old[q][r] + alpha * acc[2 * half + q][ni][r]), rs,
+                            voff[r] + (2 * half + q) * 128, 0, kCPolicy);
+            }
         }
-        const __amdgpu_buffer_rsrc_t rs = colblock_rsrc(C + (j0 + 32 * ni) * ldc + i0, ldc);
-#pragma unroll
-        for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-            for (int r = 0; r < 16; ++r)
-                __builtin_amdgcn_raw_buffer_store_b32(
-                    __builtin_bit_cast(uint32_t, beta * old[mi][r] + alpha * acc[mi][ni][r]), rs, voff[r] + mi * 128,
-                    0, kCPolicy);
     }
 }
 
@@ -416,6 +445,13 @@ k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char 
     for (; st < nks; ++st) stage(st, std::false_type{});
     // per-strip power-of-two panel scales: the products carry 2^(ex + ey)
     const float a = xsexp ? ldexpf(alpha, -(xsexp[tm] + ysexp[tn])) : alpha;
+    static_assert(sizeof(smem) >= 4 * 32768, "the LDS epilogue needs 32 KB per wave");
+    if (!kBetaZero && c_lds_ok(M, N, C, ldc, tm, tn)) {
+        stage_barrier();
+        epilogue_lds<2>(acc, a, beta, C, ldc, (int64_t)tm * OMX_STRIP + wm * 128, (int64_t)tn * OMX_STRIP + wn * 128,
+                        lane, wave, &smem[0][0][0]);
+        return;
+    }
     epilogue<kBetaZero>(acc, M, N, a, beta, C, ldc, tm, tn, wm, wn, lane);
 }
 
@@ -577,11 +613,11 @@ k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *
     }
     // beta != 0 interior tiles: the C update through the idle LDS ring (same
     // arithmetic as the register epilogue, bit-identical results; in the LU
-    // +1 %, on its own +0.5 % at k = 10240 and +5 % at k = 5120).  The FP16
-    // kernel keeps the register epilogue: there the extra live state spills.
+    // +1 %, on its own +0.5 % at k = 10240 and +5 % at k = 5120).
     if (!kBetaZero && c_lds_ok(M, N, C, ldc, tm, tn)) {
         stage_barrier();
-        epilogue_lds(acc, alpha, beta, C, ldc, (int64_t)tm * OMX_STRIP + wm * 128,
+        static_assert(sizeof(smem) >= 4 * 32768, "the LDS epilogue needs 32 KB per wave");
+        epilogue_lds<4>(acc, alpha, beta, C, ldc, (int64_t)tm * OMX_STRIP + wm * 128,
                      (int64_t)tn * OMX_STRIP + wn * 128, lane, wave, &smem[0][0][0]);
         return;
     }
