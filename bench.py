@@ -364,7 +364,7 @@ def ir_kernels(gs, seed, reps=3):
                        "fp64_frac": round(tf / PEAK_FP64_VALU_TFLOPS, 4),
                        "note": "each element is one LCG step (64-bit integer multiply-add), an int64->FP64 "
                                "conversion and scaling, and one FP64 FMA: the FP64 fraction understates the "
-                               "VALU issue rate (11 VALU instructions per element, SIMDs issuing VALU ~96 % of "
+                               "VALU issue rate (9 VALU instructions per element, SIMDs issuing VALU ~96 % of "
                                "their cycles: profiles/ir_kernels_n204800.txt)"}
     return out
 

@@ -35,6 +35,33 @@ int gemv_segments(int64_t n) {
     return (int)s;
 }
 
+// acc += A(i, j) x_j for the columns j of [j0, j1), in column order, the
+// row's LCG state s stepping by the map jn per column (s leaves as the state
+// of column j1).  Four columns per trip with a 32-bit trip count: the loop
+// test and the state copies are then scalar or amortised, leaving per
+// element the 64-bit affine step (four integer instructions), the exact
+// int64 -> FP64 conversion (three), the scaling and the FMA.
+__device__ __forceinline__ void otf_columns(uint64_t &s, double &acc, omx_map jn, const double *__restrict__ x,
+                                            int64_t j0, int64_t j1) {
+    const double *__restrict__ xp = x + j0;
+    const int cnt = (int)(j1 - j0);
+    int t = 0;
+    for (; t + 4 <= cnt; t += 4) {
+        const uint64_t s1 = omx_apply(jn, s);
+        const uint64_t s2 = omx_apply(jn, s1);
+        const uint64_t s3 = omx_apply(jn, s2);
+        acc = __builtin_fma(omx_lcg_value(s), xp[t], acc);
+        acc = __builtin_fma(omx_lcg_value(s1), xp[t + 1], acc);
+        acc = __builtin_fma(omx_lcg_value(s2), xp[t + 2], acc);
+        acc = __builtin_fma(omx_lcg_value(s3), xp[t + 3], acc);
+        s = omx_apply(jn, s3);
+    }
+    for (; t < cnt; ++t) {
+        acc = __builtin_fma(omx_lcg_value(s), xp[t], acc);
+        s = omx_apply(jn, s);
+    }
+}
+
 // Row i of the workgroup's 256-row block, columns [c0, c1): walk along the
 // row with the map a^n (one affine step per column).  The diagonal column
 // takes diag[i] instead of the LCG value (otf_gemv_diag_d,
@@ -55,23 +82,17 @@ k_otf_gemv(int64_t n, uint64_t x00, omx_map jn, const double *__restrict__ diag,
     // diagonal columns of this row block
     const int64_t d0 = (int64_t)blockIdx.x * kGemvRows;
     const int64_t d1 = d0 + kGemvRows;
+    // c0 <= lo <= hi <= c1
     const int64_t lo = c0 > d0 ? c0 : (d0 < c1 ? d0 : c1);
-    const int64_t hi = c1 < d1 ? c1 : d1;
-    int64_t j = c0;
-    for (; j < lo; ++j) {
-        acc = __builtin_fma(omx_lcg_value(s), x[j], acc);
-        s = omx_apply(jn, s);
-    }
+    const int64_t hi = c1 < d1 ? (c1 > lo ? c1 : lo) : (d1 > lo ? d1 : lo);
+    otf_columns(s, acc, jn, x, c0, lo);
     const double di = diag[i];
-    for (; j < hi; ++j) {
+    for (int64_t j = lo; j < hi; ++j) {
         const double a = (j == i) ? di : omx_lcg_value(s);
         acc = __builtin_fma(a, x[j], acc);
         s = omx_apply(jn, s);
     }
-    for (; j < c1; ++j) {
-        acc = __builtin_fma(omx_lcg_value(s), x[j], acc);
-        s = omx_apply(jn, s);
-    }
+    otf_columns(s, acc, jn, x, hi, c1);
     if (part)
         part[seg * n + i] = acc;
     else
