@@ -335,6 +335,18 @@ int omx_gemm_f16_packed_scaled(int64_t m, int64_t n, int64_t k, float alpha,
                                float beta, float *C, int64_t ldc,
                                void *stream);
 
+/* omx_gemm_f16_packed (x_row0 = y_row0 = 0) for the -alt panel products,
+ * whose inverse factor is triangular: tri = 1, Y(j, l) = 0 for l > j (the
+ * column panel times the upper-triangular U^-1, panel_trf_gdirect.hpp:100-104,
+ * :158-162); tri = 2, X(i, l) = 0 for l > i (the lower-triangular L^-1 times
+ * the row panel, :136-140, :178-182); tri = 0: all of K.  Each 256 x 256
+ * output tile stops at the end of its diagonal 256-column k tile: the
+ * products it skips are exact zeros, so C is that of omx_gemm_f16_packed
+ * with about half the MFMAs. */
+int omx_gemm_f16_packed_tri(int64_t m, int64_t n, int64_t k, float alpha,
+                            const uint16_t *X, const uint16_t *Y, int tri,
+                            float beta, float *C, int64_t ldc, void *stream);
+
 /* omx_gemm_f16_packed_scaled on K windows of wider packed panels. */
 int omx_gemm_f16_packed_window(int64_t m, int64_t n, int64_t k, float alpha,
                                const uint16_t *X, const int32_t *xsexp,

@@ -314,14 +314,21 @@ __global__ void __launch_bounds__(kThreads, 1)
 k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *__restrict__ X,
            const char *__restrict__ Y, float *__restrict__ C, int64_t ldc, int tiles_m, int tiles_n,
            const int32_t *__restrict__ xsexp, const int32_t *__restrict__ ysexp, int64_t xstrip,
-           int64_t ystrip) {
+           int64_t ystrip, int ktri) {
     __shared__ __attribute__((aligned(1024))) char smem[kNbuf][2][16384];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar tile offsets
     const int wm = wave & 1, wn = wave >> 1;
     int tm, tn;
     tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    const int nks = (int)(K / 32);
+    int nks = (int)(K / 32);
+    // a triangular factor (ktri 1: Y(j, l) = 0 for l > j; 2: X(i, l) = 0
+    // for l > i): the tile's products past its diagonal 256-column k tile
+    // are zero, so its k loop ends there (8 stages of 32 per k tile)
+    if (ktri == 1)
+        nks = min(nks, 8 * (tn + 1));
+    else if (ktri == 2)
+        nks = min(nks, 8 * (tm + 1));
     // strips are xstrip / ystrip bytes apart (the packed panels may be wider
     // than the K window this product uses)
     const char *xs = X + (int64_t)tm * xstrip;
@@ -980,14 +987,11 @@ bool window_ok(int64_t k, int64_t ldk, int64_t k0, int64_t align) {
     return ldk > 0 && ldk % align == 0 && k0 >= 0 && k0 % align == 0 && k0 + k <= ldk;
 }
 
-}  // namespace
-
-extern "C" {
-
-int omx_gemm_f16_packed_window(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X,
-                               const int32_t *xsexp, int64_t x_row0, int64_t x_ldk, int64_t x_k0,
-                               const uint16_t *Y, const int32_t *ysexp, int64_t y_row0, int64_t y_ldk,
-                               int64_t y_k0, float beta, float *C, int64_t ldc, void *stream) {
+// The FP16 products; tri as in omx_gemm_f16_packed_tri (0: full K).
+int gemm_f16_launch(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X, const int32_t *xsexp,
+                    int64_t x_row0, int64_t x_ldk, int64_t x_k0, const uint16_t *Y, const int32_t *ysexp,
+                    int64_t y_row0, int64_t y_ldk, int64_t y_k0, float beta, float *C, int64_t ldc, int tri,
+                    void *stream) {
     int rc = gemm_args(m, n, k, OMX_KSTAGE16, X, x_row0, Y, y_row0, C, ldc);
     if (rc) return rc;
     if ((xsexp == nullptr) != (ysexp == nullptr)) return OMX_ERR_ARG;
@@ -1000,6 +1004,7 @@ int omx_gemm_f16_packed_window(int64_t m, int64_t n, int64_t k, float alpha, con
     const int32_t *xe = xsexp ? xsexp + x_row0 / OMX_STRIP : nullptr;
     const int32_t *ye = ysexp ? ysexp + y_row0 / OMX_STRIP : nullptr;
     if (use_half_tiles()) {
+        // (full K also for a triangular factor: its skipped products are zero)
         const int tiles_h = (int)omx_cdiv(n, 128);
         if ((int64_t)tiles_m * tiles_h >= ((int64_t)1 << 31)) return OMX_ERR_ARG;
         dim3 grid((unsigned)(tiles_m * tiles_h));
@@ -1017,12 +1022,30 @@ int omx_gemm_f16_packed_window(int64_t m, int64_t n, int64_t k, float alpha, con
     dim3 grid((unsigned)(tiles_m * tiles_n));
     if (beta == 0.0f)
         hipLaunchKernelGGL((k_gemm_f16<true, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
-                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye, xstrip, ystrip);
+                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye, xstrip, ystrip, tri);
     else
         hipLaunchKernelGGL((k_gemm_f16<false, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
-                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye, xstrip, ystrip);
+                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye, xstrip, ystrip, tri);
     OMX_LAUNCH_CHECK();
     return OMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int omx_gemm_f16_packed_window(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X,
+                               const int32_t *xsexp, int64_t x_row0, int64_t x_ldk, int64_t x_k0,
+                               const uint16_t *Y, const int32_t *ysexp, int64_t y_row0, int64_t y_ldk,
+                               int64_t y_k0, float beta, float *C, int64_t ldc, void *stream) {
+    return gemm_f16_launch(m, n, k, alpha, X, xsexp, x_row0, x_ldk, x_k0, Y, ysexp, y_row0, y_ldk, y_k0, beta, C, ldc,
+                           0, stream);
+}
+
+int omx_gemm_f16_packed_tri(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X, const uint16_t *Y,
+                            int tri, float beta, float *C, int64_t ldc, void *stream) {
+    if (tri < 0 || tri > 2) return OMX_ERR_ARG;
+    return gemm_f16_launch(m, n, k, alpha, X, nullptr, 0, k, 0, Y, nullptr, 0, k, 0, beta, C, ldc, tri, stream);
 }
 
 int omx_gemm_f16_packed_scaled(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X,

@@ -660,16 +660,19 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
                 RC(omx_pack_f16(b, b, Wup, b, 1, s->alt == 2 ? OMX_MASK_K_LE_R : OMX_MASK_NONE, Wup_h, nullptr, cs));
                 RC(omx_pack_f16(rest, b, Acol, 1, lda, OMX_MASK_NONE, Tp, nullptr, cs));
                 RC(T.end(sp, cs, (double)(rest + b) * b));
+                // U^-1 is upper triangular: each output column tile stops
+                // at its diagonal k tile (the rest of its products are zero)
                 TIMED(T, kT_TRSM_L, cs, 2.0 * rb2,
-                      omx_gemm_f16_packed(rest, b, b, 1.0f, Tp, 0, Wup_h, 0, 0.0f, Acol, lda, cs));
+                      omx_gemm_f16_packed_tri(rest, b, b, 1.0f, Tp, Wup_h, 1, 0.0f, Acol, lda, cs));
                 // X(i, l) = W(i, l), strict lower + unit diagonal for -alt 2
                 RC(T.beg(kT_CONV_R, cs, &sp));
                 RC(omx_pack_f16(b, b, Wlo, 1, b, s->alt == 2 ? OMX_MASK_R_GT_K_UNIT : OMX_MASK_NONE, Wlo_h, nullptr,
                                 cs));
                 RC(omx_pack_f16(rest, b, Arow, lda, 1, OMX_MASK_NONE, Tp, nullptr, cs));
                 RC(T.end(sp, cs, (double)(rest + b) * b));
+                // L^-1 lower triangular: each output row tile likewise
                 TIMED(T, kT_TRSM_R, cs, 2.0 * rb2,
-                      omx_gemm_f16_packed(b, rest, b, 1.0f, Wlo_h, 0, Tp, 0, 0.0f, Arow, lda, cs));
+                      omx_gemm_f16_packed_tri(b, rest, b, 1.0f, Wlo_h, Tp, 2, 0.0f, Arow, lda, cs));
             }
             s->crit_flops += 4.0 * rest * (double)b * b;
             const int prev = (i > 0 && s->prec[i - 1] > 0) ? 1 : 0;

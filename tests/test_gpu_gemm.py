@@ -171,6 +171,41 @@ def test_gemm_f16_beta_zero_and_offsets():
     _gemm_case(1024, 1280, 512, -1.0, 1.0, x_row0=768, y_row0=256, ldc_pad=100, seed=6)
 
 
+@pytest.mark.parametrize("tri,m,n,k,beta", [(1, 1000, 768, 768, 0.0), (1, 2560, 2560, 2560, 0.0),
+                                             (1, 700, 1100, 512, 1.0), (2, 768, 1000, 768, 0.0),
+                                             (2, 2560, 2816, 2560, 0.0), (2, 1100, 700, 512, 1.0)])
+def test_gemm_f16_triangular_factor_equals_full_product(tri, m, n, k, beta):
+    """omx_gemm_f16_packed_tri against omx_gemm_f16_packed on operands that
+    hold the triangular factor's zeros (tri 1: Y(j, l) = 0 for l > j, the
+    column panel times U^-1; tri 2: X(i, l) = 0 for l > i, L^-1 times the row
+    panel): every product a tile skips is zero, so C is the same bit for
+    bit, edge tiles and tiles past the factor's order included."""
+    print(f"gemm_f16_tri tri={tri} {m}x{n}x{k} beta={beta}", flush=True)
+    torch = torch_cuda()
+    L = _L()
+    rng = np.random.default_rng(100 * tri + m)
+    Xh = rng.uniform(-1, 1, (m, k)).astype(np.float16)
+    Yh = rng.uniform(-1, 1, (n, k)).astype(np.float16)
+    if tri == 1:
+        Yh[np.arange(n)[:, None] < np.arange(k)[None, :]] = 0
+    else:
+        Xh[np.arange(m)[:, None] < np.arange(k)[None, :]] = 0
+    ldc = m + 5
+    C0 = rng.uniform(-1, 1, (n, ldc)).astype(np.float32)
+    Xd, Yd = dev(pack16(Xh).view(np.int16)), dev(pack16(Yh).view(np.int16))
+    C_full, C_tri = dev(C0.ravel()), dev(C0.ravel())
+    assert L.omx_gemm_f16_packed(m, n, k, 1.0, ptr(Xd), 0, ptr(Yd), 0, beta, ptr(C_full), ldc, None) == 0
+    assert L.omx_gemm_f16_packed_tri(m, n, k, 1.0, ptr(Xd), ptr(Yd), tri, beta, ptr(C_tri), ldc, None) == 0
+    full, got = host(C_full), host(C_tri)
+    assert np.array_equal(got, full)  # (an exact zero may differ in sign)
+    # and the product is the triangular one
+    X64, Y64 = Xh.astype(np.float64), Yh.astype(np.float64)
+    want = (beta * C0[:, :m].T.astype(np.float64) if beta else 0) + X64 @ Y64.T
+    bound = (k + 2) * 2.0**-24 * (np.abs(X64) @ np.abs(Y64).T) + 2.0**-23 * np.abs(want)
+    assert np.all(np.abs(got.reshape(n, ldc)[:, :m].T - want) <= bound)
+    assert L.omx_gemm_f16_packed_tri(m, n, k, 1.0, ptr(Xd), ptr(Yd), 3, beta, ptr(C_tri), ldc, None) == -1
+
+
 def test_gemm_f8_schur(orc):
     torch = torch_cuda()
     L = _L()
