@@ -518,11 +518,16 @@ k_tri_inv64(int nblk, const float *__restrict__ D, int64_t ldd, float *__restric
     }
 }
 
-__global__ void k_zero2(int64_t n, float *__restrict__ A, float *__restrict__ B, int64_t ld) {
-    const int64_t j = blockIdx.y;
-    float *P = blockIdx.z ? B : A;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        P[i + j * ld] = 0.0f;
+// The entries of the two inverses that k_tri_inv64 and the merges do not
+// write: in column j, Xu below and Xl above j's diagonal 64 x 64 block (the
+// diagonal blocks are written whole, the merges write every off-diagonal
+// block of the other side).  One workgroup per column.
+__global__ void k_zero_offdiag(int64_t n, float *__restrict__ Xu, float *__restrict__ Xl, int64_t ld) {
+    const int64_t j = blockIdx.x;
+    const int64_t top = (j / kNB) * kNB, bot = top + kNB;
+    float *u = Xu + j * ld, *l = Xl + j * ld;
+    for (int64_t i = bot + threadIdx.x; i < n; i += blockDim.x) u[i] = 0.0f;
+    for (int64_t i = threadIdx.x; i < top; i += blockDim.x) l[i] = 0.0f;
 }
 
 __global__ void k_identity(int64_t n, float *__restrict__ A, int64_t lda) {
@@ -731,7 +736,7 @@ int omx_trtri_lu_f32(int64_t n, const float *D, int64_t ldd, float *Xu, float *X
     if (n <= 0 || n % kNB || n > 65535 || !D || !Xu || !Xl || !work || ldd < n || ldx < n) return OMX_ERR_ARG;
     hipStream_t st = omx_stream(stream);
     const int nblk = (int)(n / kNB);
-    hipLaunchKernelGGL(k_zero2, dim3((unsigned)omx_cdiv(n, 256), (unsigned)n, 2u), dim3(256), 0, st, n, Xu, Xl, ldx);
+    hipLaunchKernelGGL(k_zero_offdiag, dim3((unsigned)n), dim3(256), 0, st, n, Xu, Xl, ldx);
     OMX_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_tri_inv64, dim3((unsigned)(2 * nblk)), dim3(256), 0, st, nblk, D, ldd, Xu, Xl, ldx);
     OMX_LAUNCH_CHECK();
