@@ -185,7 +185,8 @@ def test_trtri_lu_inverts_both_triangles(n):
     """U^-1 and L^-1 of a factored block by diagonal-block substitution and
     level-by-level merges (n / 64 = 1, 5, 10, 40 blocks: odd counts leave a
     range for a later level): against FP64 inverses of the same FP32
-    triangles, the other triangle exactly zero, L^-1's diagonal exactly 1."""
+    triangles, the other triangle exactly zero (outputs and workspace start
+    as NaN), L^-1's diagonal exactly 1."""
     torch = torch_cuda()
     L = _L()
     rng = np.random.default_rng(n)
@@ -194,9 +195,10 @@ def test_trtri_lu_inverts_both_triangles(n):
     Ut += np.diag(1.0 + np.abs(Ut).sum(axis=1))
     D = (np.tril(Lt, -1) + Ut).astype(np.float32)
     Dd, ldd = _fdev(D, n + 16)
-    Xu = empty(n * n, torch.float32)
-    Xl = empty(n * n, torch.float32)
-    W = empty(n * n, torch.float32)
+    # every entry of the outputs must be written (zeros included): start from NaN
+    Xu = empty(n * n, torch.float32).fill_(float("nan"))
+    Xl = empty(n * n, torch.float32).fill_(float("nan"))
+    W = empty(n * n, torch.float32).fill_(float("nan"))
     assert L.omx_trtri_lu_f32(n, ptr(Dd), ldd, ptr(Xu), ptr(Xl), n, ptr(W), None) == 0
     xu = colmajor_to_numpy(Xu, n, n).astype(np.float64)
     xl = colmajor_to_numpy(Xl, n, n).astype(np.float64)
