@@ -342,10 +342,15 @@ int omx_gemm_f16_packed_scaled(int64_t m, int64_t n, int64_t k, float alpha,
  * the row panel, :136-140, :178-182); tri = 0: all of K.  Each 256 x 256
  * output tile stops at the end of its diagonal 256-column k tile: the
  * products it skips are exact zeros, so C is that of omx_gemm_f16_packed
- * with about half the MFMAs. */
+ * with about half the MFMAs.  tile_max (beta = 0 only; NULL: none)
+ * receives each 256 x 256 tile's max |C| as omx_panel_tile_maxabs stores it
+ * for the panel C is: C's rows are the panel's rows for tri 0 and 1, C's
+ * columns for tri 2 (the row panel); the same values, without a pass over
+ * C. */
 int omx_gemm_f16_packed_tri(int64_t m, int64_t n, int64_t k, float alpha,
                             const uint16_t *X, const uint16_t *Y, int tri,
-                            float beta, float *C, int64_t ldc, void *stream);
+                            float beta, float *C, int64_t ldc,
+                            float *tile_max, void *stream);
 
 /* omx_gemm_f16_packed_scaled on K windows of wider packed panels. */
 int omx_gemm_f16_packed_window(int64_t m, int64_t n, int64_t k, float alpha,

@@ -59,7 +59,7 @@ SIGNATURES = {
     "omx_panel_tile_maxabs": (i32, [i64, i64, vp, i64, i64, vp, vp]),
     "omx_gemm_f16_packed": (i32, [i64, i64, i64, flt, vp, i64, vp, i64, flt, vp, i64, vp]),
     "omx_gemm_f16_packed_scaled": (i32, [i64, i64, i64, flt, vp, vp, i64, vp, vp, i64, flt, vp, i64, vp]),
-    "omx_gemm_f16_packed_tri": (i32, [i64, i64, i64, flt, vp, vp, i32, flt, vp, i64, vp]),
+    "omx_gemm_f16_packed_tri": (i32, [i64, i64, i64, flt, vp, vp, i32, flt, vp, i64, vp, vp]),
     "omx_gemm_f16_packed_window": (i32, [i64, i64, i64, flt, vp, vp, i64, i64, i64, vp, vp, i64, i64, i64, flt, vp,
                                          i64, vp]),
     "omx_gemm_f8_packed_window": (i32, [i64, i64, i64, flt, vp, vp, i64, i64, i64, vp, vp, i64, i64, i64, flt, vp,

@@ -116,10 +116,11 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t colblock_rsrc(float *p, int64_
     return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)(ldc * 32 * 4), 0x00020000);
 }
 
+// mx (beta = 0): max |value stored| by this lane, folded into what it holds.
 template <bool kBetaZero>
 __device__ __forceinline__ void epilogue(floatx16 (&acc)[4][4], int64_t M, int64_t N, float alpha,
                                          float beta, float *__restrict__ C, int64_t ldc, int tm,
-                                         int tn, int wm, int wn, int lane) {
+                                         int tn, int wm, int wn, int lane, float &mx) {
     const int64_t i0 = (int64_t)tm * OMX_STRIP + wm * 128;
     const int64_t j0 = (int64_t)tn * OMX_STRIP + wn * 128;
     if ((int64_t)(tm + 1) * OMX_STRIP <= M && (int64_t)(tn + 1) * OMX_STRIP <= N &&
@@ -135,9 +136,12 @@ __device__ __forceinline__ void epilogue(floatx16 (&acc)[4][4], int64_t M, int64
 #pragma unroll
                 for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-                    for (int r = 0; r < 16; ++r)
-                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, alpha * acc[mi][ni][r]),
-                                                              rs, voff[r] + mi * 128, 0, kCPolicy);
+                    for (int r = 0; r < 16; ++r) {
+                        const float v = alpha * acc[mi][ni][r];
+                        mx = fmaxf(mx, fabsf(v));
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, voff[r] + mi * 128,
+                                                              0, kCPolicy);
+                    }
             }
             return;
         }
@@ -193,7 +197,13 @@ __device__ __forceinline__ void epilogue(floatx16 (&acc)[4][4], int64_t M, int64
                 const int64_t j = jb + ni * 32 + 8 * (r >> 2) + (r & 3);
                 if (j < N) {
                     const float v = acc[mi][ni][r];
-                    cp[j * ldc] = kBetaZero ? alpha * v : beta * old[r] + alpha * v;
+                    if (kBetaZero) {
+                        const float w = alpha * v;
+                        mx = fmaxf(mx, fabsf(w));
+                        cp[j * ldc] = w;
+                    } else {
+                        cp[j * ldc] = beta * old[r] + alpha * v;
+                    }
                 }
             }
         }
@@ -309,12 +319,12 @@ __device__ __forceinline__ void epilogue_lds(floatx16 (&acc)[4][4], float alpha,
 // previous stage and reads the next stage's first fragments, which then
 // arrive while the second 16 MFMAs run.
 // ---------------------------------------------------------------------------
-template <bool kBetaZero, int kNbuf>
+template <bool kBetaZero, int kNbuf, bool kTmax = false>
 __global__ void __launch_bounds__(kThreads, 1)
 k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *__restrict__ X,
            const char *__restrict__ Y, float *__restrict__ C, int64_t ldc, int tiles_m, int tiles_n,
            const int32_t *__restrict__ xsexp, const int32_t *__restrict__ ysexp, int64_t xstrip,
-           int64_t ystrip, int ktri) {
+           int64_t ystrip, int ktri, float *__restrict__ tmax, int tmax_tr) {
     __shared__ __attribute__((aligned(1024))) char smem[kNbuf][2][16384];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar tile offsets
@@ -459,7 +469,22 @@ k_gemm_f16(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char 
                         lane, wave, &smem[0][0][0]);
         return;
     }
-    epilogue<kBetaZero>(acc, M, N, a, beta, C, ldc, tm, tn, wm, wn, lane);
+    float mx = 0.0f;
+    epilogue<kBetaZero>(acc, M, N, a, beta, C, ldc, tm, tn, wm, wn, lane, mx);
+    if constexpr (kBetaZero && kTmax) {
+        // the tile's max |C| (what omx_panel_tile_maxabs finds there): lanes,
+        // then the four waves through the idle ring (once every wave is past
+        // its last ring read).  Without kTmax the maxima above are dead code.
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        float *red = reinterpret_cast<float *>(&smem[0][0][0]);
+        __syncthreads();
+        if (lane == 0) red[wave] = mx;
+        __syncthreads();
+        if (tid == 0)
+            tmax[tmax_tr ? (int64_t)tn * tiles_m + tm : (int64_t)tm * tiles_n + tn] =
+                fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -628,7 +653,8 @@ k_gemm_f8(int64_t M, int64_t N, int64_t K, float alpha, float beta, const char *
                      (int64_t)tn * OMX_STRIP + wn * 128, lane, wave, &smem[0][0][0]);
         return;
     }
-    epilogue<kBetaZero>(acc, M, N, alpha, beta, C, ldc, tm, tn, wm, wn, lane);
+    float mx = 0.0f;  // (unused here)
+    epilogue<kBetaZero>(acc, M, N, alpha, beta, C, ldc, tm, tn, wm, wn, lane, mx);
 }
 
 // ---------------------------------------------------------------------------
@@ -987,11 +1013,12 @@ bool window_ok(int64_t k, int64_t ldk, int64_t k0, int64_t align) {
     return ldk > 0 && ldk % align == 0 && k0 >= 0 && k0 % align == 0 && k0 + k <= ldk;
 }
 
-// The FP16 products; tri as in omx_gemm_f16_packed_tri (0: full K).
+// The FP16 products; tri and tile_max as in omx_gemm_f16_packed_tri (0 and
+// NULL: full K, no tile maxima).
 int gemm_f16_launch(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X, const int32_t *xsexp,
                     int64_t x_row0, int64_t x_ldk, int64_t x_k0, const uint16_t *Y, const int32_t *ysexp,
                     int64_t y_row0, int64_t y_ldk, int64_t y_k0, float beta, float *C, int64_t ldc, int tri,
-                    void *stream) {
+                    float *tile_max, void *stream) {
     int rc = gemm_args(m, n, k, OMX_KSTAGE16, X, x_row0, Y, y_row0, C, ldc);
     if (rc) return rc;
     if ((xsexp == nullptr) != (ysexp == nullptr)) return OMX_ERR_ARG;
@@ -1003,7 +1030,7 @@ int gemm_f16_launch(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t
     const char *yp = reinterpret_cast<const char *>(Y) + (y_row0 / OMX_STRIP) * ystrip + (y_k0 / OMX_KSTAGE16) * kChunk;
     const int32_t *xe = xsexp ? xsexp + x_row0 / OMX_STRIP : nullptr;
     const int32_t *ye = ysexp ? ysexp + y_row0 / OMX_STRIP : nullptr;
-    if (use_half_tiles()) {
+    if (use_half_tiles() && !tile_max) {
         // (full K also for a triangular factor: its skipped products are zero)
         const int tiles_h = (int)omx_cdiv(n, 128);
         if ((int64_t)tiles_m * tiles_h >= ((int64_t)1 << 31)) return OMX_ERR_ARG;
@@ -1020,12 +1047,16 @@ int gemm_f16_launch(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t
         return OMX_OK;
     }
     dim3 grid((unsigned)(tiles_m * tiles_n));
-    if (beta == 0.0f)
+    if (beta == 0.0f && tile_max)
+        hipLaunchKernelGGL((k_gemm_f16<true, kNbuf, true>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k,
+                           alpha, beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye, xstrip, ystrip, tri, tile_max,
+                           tri == 2 ? 1 : 0);
+    else if (beta == 0.0f)
         hipLaunchKernelGGL((k_gemm_f16<true, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
-                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye, xstrip, ystrip, tri);
+                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye, xstrip, ystrip, tri, nullptr, 0);
     else
         hipLaunchKernelGGL((k_gemm_f16<false, kNbuf>), grid, dim3(kThreads), 0, omx_stream(stream), m, n, k, alpha,
-                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye, xstrip, ystrip, tri);
+                           beta, xp, yp, C, ldc, tiles_m, tiles_n, xe, ye, xstrip, ystrip, tri, nullptr, 0);
     OMX_LAUNCH_CHECK();
     return OMX_OK;
 }
@@ -1039,13 +1070,14 @@ int omx_gemm_f16_packed_window(int64_t m, int64_t n, int64_t k, float alpha, con
                                const uint16_t *Y, const int32_t *ysexp, int64_t y_row0, int64_t y_ldk,
                                int64_t y_k0, float beta, float *C, int64_t ldc, void *stream) {
     return gemm_f16_launch(m, n, k, alpha, X, xsexp, x_row0, x_ldk, x_k0, Y, ysexp, y_row0, y_ldk, y_k0, beta, C, ldc,
-                           0, stream);
+                           0, nullptr, stream);
 }
 
 int omx_gemm_f16_packed_tri(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X, const uint16_t *Y,
-                            int tri, float beta, float *C, int64_t ldc, void *stream) {
-    if (tri < 0 || tri > 2) return OMX_ERR_ARG;
-    return gemm_f16_launch(m, n, k, alpha, X, nullptr, 0, k, 0, Y, nullptr, 0, k, 0, beta, C, ldc, tri, stream);
+                            int tri, float beta, float *C, int64_t ldc, float *tile_max, void *stream) {
+    if (tri < 0 || tri > 2 || (tile_max && beta != 0.0f)) return OMX_ERR_ARG;
+    return gemm_f16_launch(m, n, k, alpha, X, nullptr, 0, k, 0, Y, nullptr, 0, k, 0, beta, C, ldc, tri, tile_max,
+                           stream);
 }
 
 int omx_gemm_f16_packed_scaled(int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *X,

@@ -524,14 +524,17 @@ static int pack_panel(omx_solver *s, int64_t i, int have_max, int64_t rest, cons
 }
 
 // Statistics of step k's finished panels (FP8 solvers): tile maxima of the
-// column and row panels and the 16 scheduler features; with a policy, the
-// MLP's outputs are copied to pinned host memory behind s->ev_dec.
+// column and row panels (unless the panel products left them: have_max)
+// and the 16 scheduler features; with a policy, the MLP's outputs are
+// copied to pinned host memory behind s->ev_dec.
 static int step_features(omx_solver *s, int64_t k, int64_t rest, const float *Akk, const float *Acol,
-                         const float *Arow, int64_t lda, int prev, bool infer, hipStream_t cs) {
+                         const float *Arow, int64_t lda, int prev, bool infer, bool have_max, hipStream_t cs) {
     const int64_t b = s->b;
     float *lmax = static_cast<float *>(s->L8max.p), *rmax = static_cast<float *>(s->R8max.p);
-    RC(omx_panel_tile_maxabs(rest, b, Acol, 1, lda, lmax, cs));
-    RC(omx_panel_tile_maxabs(rest, b, Arow, lda, 1, rmax, cs));
+    if (!have_max) {
+        RC(omx_panel_tile_maxabs(rest, b, Acol, 1, lda, lmax, cs));
+        RC(omx_panel_tile_maxabs(rest, b, Arow, lda, 1, rmax, cs));
+    }
     const int64_t nt = (int64_t)omx_panel_exps_count(rest, b);
     float *f = static_cast<float *>(s->feat.p) + k * OMX_MLP_NFEATURES;
     RC(omx_sched_features_f32(nt, lmax, nt, rmax, Akk, lda, b, k, s->nb, s->n, prev, f, cs));
@@ -639,6 +642,7 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
             TIMED(T, kT_DIAG_LU, cs, 2.0 / 3.0 * b * (double)b * b, omx_getrf_nopiv_f32(b, Akk, lda, info + i, cs));
         int choice = 0;
         if (rest > 0) {
+            bool have_max = false;  // the panels' tile maxima already computed
             float *Acol = A + (ib + b) + ib * lda;
             float *Arow = A + ib + (ib + b) * lda;
             const double rb2 = (double)rest * b * b;
@@ -662,8 +666,12 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
                 RC(T.end(sp, cs, (double)(rest + b) * b));
                 // U^-1 is upper triangular: each output column tile stops
                 // at its diagonal k tile (the rest of its products are zero)
+                // with the statistics on, the product also leaves the
+                // column panel's tile maxima (no separate pass over it)
+                float *lmax = stats ? static_cast<float *>(s->L8max.p) : nullptr;
+                float *rmax = stats ? static_cast<float *>(s->R8max.p) : nullptr;
                 TIMED(T, kT_TRSM_L, cs, 2.0 * rb2,
-                      omx_gemm_f16_packed_tri(rest, b, b, 1.0f, Tp, Wup_h, 1, 0.0f, Acol, lda, cs));
+                      omx_gemm_f16_packed_tri(rest, b, b, 1.0f, Tp, Wup_h, 1, 0.0f, Acol, lda, lmax, cs));
                 // X(i, l) = W(i, l), strict lower + unit diagonal for -alt 2
                 RC(T.beg(kT_CONV_R, cs, &sp));
                 RC(omx_pack_f16(b, b, Wlo, 1, b, s->alt == 2 ? OMX_MASK_R_GT_K_UNIT : OMX_MASK_NONE, Wlo_h, nullptr,
@@ -672,11 +680,12 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
                 RC(T.end(sp, cs, (double)(rest + b) * b));
                 // L^-1 lower triangular: each output row tile likewise
                 TIMED(T, kT_TRSM_R, cs, 2.0 * rb2,
-                      omx_gemm_f16_packed_tri(b, rest, b, 1.0f, Wlo_h, Tp, 2, 0.0f, Arow, lda, cs));
+                      omx_gemm_f16_packed_tri(b, rest, b, 1.0f, Wlo_h, Tp, 2, 0.0f, Arow, lda, rmax, cs));
+                have_max = stats;
             }
             s->crit_flops += 4.0 * rest * (double)b * b;
             const int prev = (i > 0 && s->prec[i - 1] > 0) ? 1 : 0;
-            if (stats) RC(step_features(s, i, rest, Akk, Acol, Arow, lda, prev, learned, cs));
+            if (stats) RC(step_features(s, i, rest, Akk, Acol, Arow, lda, prev, learned, have_max, cs));
             // with a policy the host waits here for the step's outputs; the
             // current pair's trailing update is already queued on the bulk
             // stream, so the GPU stays busy

@@ -195,7 +195,7 @@ def test_gemm_f16_triangular_factor_equals_full_product(tri, m, n, k, beta):
     Xd, Yd = dev(pack16(Xh).view(np.int16)), dev(pack16(Yh).view(np.int16))
     C_full, C_tri = dev(C0.ravel()), dev(C0.ravel())
     assert L.omx_gemm_f16_packed(m, n, k, 1.0, ptr(Xd), 0, ptr(Yd), 0, beta, ptr(C_full), ldc, None) == 0
-    assert L.omx_gemm_f16_packed_tri(m, n, k, 1.0, ptr(Xd), ptr(Yd), tri, beta, ptr(C_tri), ldc, None) == 0
+    assert L.omx_gemm_f16_packed_tri(m, n, k, 1.0, ptr(Xd), ptr(Yd), tri, beta, ptr(C_tri), ldc, None, None) == 0
     full, got = host(C_full), host(C_tri)
     assert np.array_equal(got, full)  # (an exact zero may differ in sign)
     # and the product is the triangular one
@@ -203,7 +203,39 @@ def test_gemm_f16_triangular_factor_equals_full_product(tri, m, n, k, beta):
     want = (beta * C0[:, :m].T.astype(np.float64) if beta else 0) + X64 @ Y64.T
     bound = (k + 2) * 2.0**-24 * (np.abs(X64) @ np.abs(Y64).T) + 2.0**-23 * np.abs(want)
     assert np.all(np.abs(got.reshape(n, ldc)[:, :m].T - want) <= bound)
-    assert L.omx_gemm_f16_packed_tri(m, n, k, 1.0, ptr(Xd), ptr(Yd), 3, beta, ptr(C_tri), ldc, None) == -1
+    assert L.omx_gemm_f16_packed_tri(m, n, k, 1.0, ptr(Xd), ptr(Yd), 3, beta, ptr(C_tri), ldc, None, None) == -1
+
+
+@pytest.mark.parametrize("tri,m,n", [(1, 1000, 768), (1, 2560, 2560), (2, 768, 1000), (2, 2560, 2816), (0, 700, 512)])
+def test_gemm_f16_tri_tile_maxima_equal_the_panel_pass(tri, m, n):
+    """The tile maxima the beta = 0 product leaves (tile_max) against
+    omx_panel_tile_maxabs over the C it wrote, bit for bit: C's rows are the
+    panel's rows for tri 0 and 1, its columns for tri 2 (the row panel);
+    edge tiles take only the entries inside C."""
+    print(f"gemm_f16_tri tile maxima tri={tri} {m}x{n}", flush=True)
+    torch = torch_cuda()
+    L = _L()
+    k = 512
+    rng = np.random.default_rng(7 * m + n + tri)
+    Xh = (rng.standard_normal((m, k)) * rng.uniform(0.1, 4.0, (m, 1))).astype(np.float16)
+    Yh = rng.uniform(-1, 1, (n, k)).astype(np.float16)
+    if tri == 1:
+        Yh[np.arange(n)[:, None] < np.arange(k)[None, :]] = 0
+    elif tri == 2:
+        Xh[np.arange(m)[:, None] < np.arange(k)[None, :]] = 0
+    ldc = m + 3
+    Xd, Yd = dev(pack16(Xh).view(np.int16)), dev(pack16(Yh).view(np.int16))
+    Cd = dev(np.full((n, ldc), np.nan, np.float32).ravel())
+    rows, kk, rs, ls = (m, n, 1, ldc) if tri != 2 else (n, m, ldc, 1)
+    cnt = L.omx_panel_exps_count(rows, kk)
+    got = empty(cnt, torch.float32).fill_(float("nan"))
+    want = empty(cnt, torch.float32)
+    assert L.omx_gemm_f16_packed_tri(m, n, k, -0.5, ptr(Xd), ptr(Yd), tri, 0.0, ptr(Cd), ldc, ptr(got), None) == 0
+    assert L.omx_panel_tile_maxabs(rows, kk, ptr(Cd), rs, ls, ptr(want), None) == 0
+    assert np.array_equal(host(got), host(want))
+    assert np.all(host(got) > 0)
+    # tile maxima need beta = 0
+    assert L.omx_gemm_f16_packed_tri(m, n, k, 1.0, ptr(Xd), ptr(Yd), tri, 1.0, ptr(Cd), ldc, ptr(got), None) == -1
 
 
 def test_gemm_f8_schur(orc):
