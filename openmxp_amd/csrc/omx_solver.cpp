@@ -652,8 +652,20 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
                 // FP32 panels by TRSM: step 0 (the reference's), -alt 0, and
                 // the FP32 Schur class (whose products take the panels as
                 // they are, so the FP16 inverse products would limit them)
-                TIMED(T, kT_TRSM_L, cs, rb2, omx_trsm_f32(OMX_TRSM_RUNN, rest, b, Akk, lda, Acol, lda, cs));
-                TIMED(T, kT_TRSM_R, cs, rb2, omx_trsm_f32(OMX_TRSM_LLNU, b, rest, Akk, lda, Arow, lda, cs));
+                if (i == 0) {
+                    // nothing is queued on the bulk stream before the first
+                    // group's panels: the two independent solves run side by
+                    // side (its look-ahead events are recorded afresh later)
+                    OMX_HIP_CHECK(hipEventRecord(s->ev_la[0], cs));
+                    OMX_HIP_CHECK(hipStreamWaitEvent(bs, s->ev_la[0], 0));
+                    TIMED(T, kT_TRSM_R, bs, rb2, omx_trsm_f32(OMX_TRSM_LLNU, b, rest, Akk, lda, Arow, lda, bs));
+                    TIMED(T, kT_TRSM_L, cs, rb2, omx_trsm_f32(OMX_TRSM_RUNN, rest, b, Akk, lda, Acol, lda, cs));
+                    OMX_HIP_CHECK(hipEventRecord(s->ev_la[1], bs));
+                    OMX_HIP_CHECK(hipStreamWaitEvent(cs, s->ev_la[1], 0));
+                } else {
+                    TIMED(T, kT_TRSM_L, cs, rb2, omx_trsm_f32(OMX_TRSM_RUNN, rest, b, Akk, lda, Acol, lda, cs));
+                    TIMED(T, kT_TRSM_R, cs, rb2, omx_trsm_f32(OMX_TRSM_LLNU, b, rest, Akk, lda, Arow, lda, cs));
+                }
             } else {
                 // U^-1 and L^-1 of the factored diagonal block
                 TIMED(T, kT_DIAG_LU, cs, 2.0 / 3.0 * b * (double)b * b,
