@@ -655,7 +655,8 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
                 if (i == 0) {
                     // nothing is queued on the bulk stream before the first
                     // group's panels: the two independent solves run side by
-                    // side (its look-ahead events are recorded afresh later)
+                    // side (the two look-ahead events borrowed here are
+                    // recorded afresh before anything waits on them again)
                     OMX_HIP_CHECK(hipEventRecord(s->ev_la[0], cs));
                     OMX_HIP_CHECK(hipStreamWaitEvent(bs, s->ev_la[0], 0));
                     TIMED(T, kT_TRSM_R, bs, rb2, omx_trsm_f32(OMX_TRSM_LLNU, b, rest, Akk, lda, Arow, lda, bs));
@@ -677,9 +678,9 @@ int omx_lu_factor(omx_solver *s, float *A, int64_t lda, const int *sched, double
                 RC(omx_pack_f16(rest, b, Acol, 1, lda, OMX_MASK_NONE, Tp, nullptr, cs));
                 RC(T.end(sp, cs, (double)(rest + b) * b));
                 // U^-1 is upper triangular: each output column tile stops
-                // at its diagonal k tile (the rest of its products are zero)
-                // with the statistics on, the product also leaves the
-                // column panel's tile maxima (no separate pass over it)
+                // at its diagonal k tile (the rest of its products are
+                // zero).  With the statistics on, the product also leaves
+                // the column panel's tile maxima (no separate pass over it).
                 float *lmax = stats ? static_cast<float *>(s->L8max.p) : nullptr;
                 float *rmax = stats ? static_cast<float *>(s->R8max.p) : nullptr;
                 TIMED(T, kT_TRSM_L, cs, 2.0 * rb2,
