@@ -68,3 +68,14 @@ def test_timer_items_are_the_references():
     buf = (ctypes.c_double * _lib.TIMER_ITEMS)()
     assert L.omx_solver_timers(None, buf, buf) < 0
 
+
+
+def test_triangular_product_rejects_bad_arguments_before_any_launch():
+    """omx_gemm_f16_packed_tri refuses an unknown tri and tile maxima with
+    beta != 0 before it touches the GPU (so this runs without one)."""
+    from openmxp_amd import _lib
+    L = _lib.lib()
+    p = 16  # never dereferenced: the checks return first
+    assert L.omx_gemm_f16_packed_tri(256, 256, 256, 1.0, p, p, 3, 0.0, p, 256, None, None) == -1
+    assert L.omx_gemm_f16_packed_tri(256, 256, 256, 1.0, p, p, -1, 0.0, p, 256, None, None) == -1
+    assert L.omx_gemm_f16_packed_tri(256, 256, 256, 1.0, p, p, 1, 1.0, p, 256, p, None) == -1
