@@ -288,6 +288,14 @@ __device__ __forceinline__ void diag_solve(const float *__restrict__ A, int64_t 
 #pragma unroll
             for (int q = 0; q < kSB; ++q) w[q] = Drow[(int64_t)min(kSB * t + q, dim - 1) * lda];
             if (kUpper) ud = Drow[(int64_t)min(i, dim - 1) * lda];
+            // lower: multipliers outside the triangle or past the block are
+            // zeroed now, so that the chain's dependent steps carry no
+            // selects (in the upper solve this spills; it selects in the
+            // chain)
+            if (!kUpper) {
+#pragma unroll
+                for (int q = 0; q < kSB; ++q) w[q] = (lane > q && i < dim) ? w[q] : 0.0f;
+            }
         } else {
             // source sub-block s at slot s (lower, s < t) or s - 1 (upper, s > t)
 #pragma unroll
@@ -330,8 +338,11 @@ __device__ __forceinline__ void diag_solve(const float *__restrict__ A, int64_t 
             } else {
 #pragma unroll
                 for (int q = 0; q < kSB; ++q) {
-                    const double m = (lane > q && i < dim) ? (double)w[q] : 0.0;
-                    v = __builtin_fma(-bcast_lane(v, q), m, v);
+                    // converted in place (hoisting all 64 conversions would
+                    // take 128 registers)
+                    float wq = w[q];
+                    asm volatile("" : "+v"(wq));
+                    v = __builtin_fma(-bcast_lane(v, q), (double)wq, v);
                 }
             }
             if (i < dim) sh.xs[i] = v;
