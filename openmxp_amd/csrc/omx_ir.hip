@@ -320,6 +320,15 @@ __device__ __forceinline__ void diag_solve(const float *__restrict__ A, int64_t 
             double v = (i < dim) ? sh.xs[i] : 0.0;
             if (fed) v -= ((sh.pp[st][0][lane] + sh.pp[st][1][lane]) + sh.pp[st][2][lane]) + sh.pp[st][3][lane];
             if (kUpper) {
+                // the multipliers outside the triangle or past the block are
+                // zeroed in place first (so the chain's steps carry no
+                // selects; masking them with the loads spills here)
+#pragma unroll
+                for (int q = 0; q < kSB; ++q) {
+                    float wq = (lane < q && c0 + q < dim) ? w[q] : 0.0f;
+                    asm volatile("" : "+v"(wq));
+                    w[q] = wq;
+                }
                 const double u = (i < dim) ? (double)ud : 1.0;
                 const double y = 1.0 / u;
                 auto quot = [&](double r) {
@@ -330,10 +339,7 @@ __device__ __forceinline__ void diag_solve(const float *__restrict__ A, int64_t 
                 // rows above subtract x_q u(i, q).  Lane q's own v is
                 // untouched from then on, so its final quotient equals x_q.
 #pragma unroll
-                for (int q = kSB - 1; q >= 0; --q) {
-                    const double m = (lane < q && c0 + q < dim) ? (double)w[q] : 0.0;
-                    v = __builtin_fma(-bcast_lane(quot(v), q), m, v);
-                }
+                for (int q = kSB - 1; q >= 0; --q) v = __builtin_fma(-bcast_lane(quot(v), q), (double)w[q], v);
                 v = quot(v);
             } else {
 #pragma unroll
